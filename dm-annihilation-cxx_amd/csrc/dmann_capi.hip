@@ -1,0 +1,309 @@
+// C ABI of libdmann.so (declared in include/dmann.h).
+//
+// Replaces the module-global device API of the reference render path
+// (src/chealpixrender.cu:28-125) with an opaque, re-entrant context.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "dmann.h"
+#include "dmann_footprint.h"
+#include "dmann_geom.h"
+#include "dmann_kernels.h"
+
+using namespace dmann;
+
+struct dmann_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int acc = kAccF32;
+    DevParams P{};
+    int64_t max_parts = 0;
+    int64_t npix = 0;
+    Particle *d_parts = nullptr; // staging for host particles
+    DiscRec *d_recs = nullptr;
+    unsigned *d_counts = nullptr;
+    unsigned long long *d_stats = nullptr;
+    MapRef map{nullptr, nullptr, nullptr};
+    double *d_out = nullptr; // finalised FP64 map for host reads
+    LaunchCfg cfg{};
+    char msg[512] = {0};
+};
+
+static char g_create_msg[512] = "";
+
+static int fail(dmann_ctx *c, int code, const char *fmt, ...)
+{
+    char *dst = c ? c->msg : g_create_msg;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(dst, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return fail((ctx), DMANN_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static void free_all(dmann_ctx *c)
+{
+    hipFree(c->d_parts);
+    hipFree(c->d_recs);
+    hipFree(c->d_counts);
+    hipFree(c->d_stats);
+    hipFree(c->map.f32);
+    hipFree(c->map.lo);
+    hipFree(c->map.f64);
+    hipFree(c->d_out);
+    c->d_parts = nullptr;
+    c->d_recs = nullptr;
+    c->d_counts = nullptr;
+    c->d_stats = nullptr;
+    c->map = MapRef{nullptr, nullptr, nullptr};
+    c->d_out = nullptr;
+}
+
+extern "C" {
+
+int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int device)
+{
+    if (!out || !p) return fail(nullptr, DMANN_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->nside < 1 || p->nside > kMaxNside)
+        return fail(nullptr, DMANN_ERR_ARG, "nside %d outside 1..%d", p->nside, kMaxNside);
+    if (max_parts < 0 || max_parts > (int64_t)INT32_MAX * 2)
+        return fail(nullptr, DMANN_ERR_ARG, "max_parts %lld outside 0..%lld", (long long)max_parts,
+                    (long long)INT32_MAX * 2);
+    if (p->precision < DMANN_PREC_FP32 || p->precision > DMANN_PREC_FP64)
+        return fail(nullptr, DMANN_ERR_ARG, "unknown precision %d", p->precision);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, DMANN_ERR_HIP, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(nullptr, DMANN_ERR_ARG, "device %d of %d", device, ndev);
+
+    dmann_ctx *c = new (std::nothrow) dmann_ctx();
+    if (!c) return fail(nullptr, DMANN_ERR_ARG, "out of host memory");
+    c->device = device;
+    c->acc = p->precision == DMANN_PREC_FP32 ? kAccF32 : p->precision == DMANN_PREC_COMPENSATED ? kAccComp : kAccF64;
+    c->max_parts = max_parts;
+    c->npix = npix_of(p->nside);
+    DevParams &P = c->P;
+    for (int i = 0; i < 3; i++) P.opos[i] = p->opos[i];
+    for (int i = 0; i < 9; i++) P.rot[i] = p->rotmatrix[i];
+    P.flux_factor = p->flux_factor;
+    P.mass_cut = p->mass_cut;
+    P.nside = p->nside;
+    P.npix = (int)c->npix;
+    P.dOmega = 4 * kPi / (double)c->npix;              // Map::setNside, datatypes.h:248-253
+    P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592)); // src/genmap.cpp:38
+    P.large_pix = 1 << 16;
+
+    hipError_t e = hipSetDevice(device);
+    int dev_cus = 256;
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            dev_cus = prop.multiProcessorCount;
+    }
+    c->cfg.prepass_blocks = dev_cus * 64;
+    c->cfg.medium_blocks = dev_cus * 8;
+    c->cfg.large_blocks = dev_cus * 4;
+    auto bail = [&](const char *what, hipError_t err) {
+        snprintf(g_create_msg, sizeof g_create_msg, "%s: %s", what, hipGetErrorString(err));
+        free_all(c);
+        delete c;
+        return DMANN_ERR_HIP;
+    };
+    if (e != hipSuccess) return bail("hipSetDevice", e);
+    const int64_t np = c->npix;
+    if (c->acc == kAccF64) {
+        if ((e = hipMalloc(&c->map.f64, np * sizeof(double))) != hipSuccess) return bail("hipMalloc map", e);
+    } else {
+        if ((e = hipMalloc(&c->map.f32, np * sizeof(float))) != hipSuccess) return bail("hipMalloc map", e);
+        if (c->acc == kAccComp && (e = hipMalloc(&c->map.lo, np * sizeof(float))) != hipSuccess)
+            return bail("hipMalloc map lo", e);
+    }
+    if ((e = hipMalloc(&c->d_recs, (max_parts ? max_parts : 1) * sizeof(DiscRec))) != hipSuccess)
+        return bail("hipMalloc records", e);
+    if ((e = hipMalloc(&c->d_counts, 4 * sizeof(unsigned))) != hipSuccess) return bail("hipMalloc counts", e);
+    if ((e = hipMalloc(&c->d_stats, kNumStats * sizeof(unsigned long long))) != hipSuccess)
+        return bail("hipMalloc stats", e);
+    if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);
+    *out = c;
+    return DMANN_OK;
+}
+
+int dmann_set_stream(dmann_ctx *c, void *stream)
+{
+    if (!c) return DMANN_ERR_ARG;
+    c->stream = (hipStream_t)stream;
+    return DMANN_OK;
+}
+
+int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (n < 0) return fail(c, DMANN_ERR_ARG, "negative particle count");
+    if (n == 0) return DMANN_OK;
+    if (!dev_parts) return fail(c, DMANN_ERR_ARG, "null particle pointer");
+    if (n > c->max_parts)
+        return fail(c, DMANN_ERR_CAPACITY, "%lld particles exceed the context capacity %lld", (long long)n,
+                    (long long)c->max_parts);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_render(c->acc, reinterpret_cast<const Particle *>(dev_parts), n, c->P, c->d_recs,
+                            c->d_counts, c->map, c->d_stats, c->cfg, c->stream));
+    return DMANN_OK;
+}
+
+int dmann_render(dmann_ctx *c, const dmann_particle *host_parts, int64_t n)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (n < 0) return fail(c, DMANN_ERR_ARG, "negative particle count");
+    if (n == 0) return DMANN_OK;
+    if (!host_parts) return fail(c, DMANN_ERR_ARG, "null particle pointer");
+    if (n > c->max_parts)
+        return fail(c, DMANN_ERR_CAPACITY, "%lld particles exceed the context capacity %lld", (long long)n,
+                    (long long)c->max_parts);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->d_parts) HIPCHK(c, hipMalloc(&c->d_parts, c->max_parts * sizeof(Particle)));
+    HIPCHK(c, hipMemcpyAsync(c->d_parts, host_parts, n * sizeof(Particle), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_render(c->acc, c->d_parts, n, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg,
+                            c->stream));
+    // the staging buffer is reused by the next call: wait for this one
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DMANN_OK;
+}
+
+int dmann_synchronize(dmann_ctx *c)
+{
+    if (!c) return DMANN_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long bad = 0;
+    HIPCHK(c, hipMemcpy(&bad, c->d_stats + kStatBad, sizeof bad, hipMemcpyDeviceToHost));
+    if (bad)
+        return fail(c, DMANN_ERR_DOMAIN, "%llu particle(s) with undefined direction (d == 0 or |cos theta| > 1)",
+                    bad);
+    return DMANN_OK;
+}
+
+int dmann_get_map_device(dmann_ctx *c, double *dev_map)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (!dev_map) return fail(c, DMANN_ERR_ARG, "null map pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_finalize(c->acc, c->map, c->npix, c->P.dOmega, dev_map, c->stream));
+    return DMANN_OK;
+}
+
+int dmann_get_map(dmann_ctx *c, double *host_map)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (!host_map) return fail(c, DMANN_ERR_ARG, "null map pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->d_out) HIPCHK(c, hipMalloc(&c->d_out, c->npix * sizeof(double)));
+    HIPCHK(c, launch_finalize(c->acc, c->map, c->npix, c->P.dOmega, c->d_out, c->stream));
+    HIPCHK(c, hipMemcpyAsync(host_map, c->d_out, c->npix * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return dmann_synchronize(c);
+}
+
+int dmann_reset(dmann_ctx *c)
+{
+    if (!c) return DMANN_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_clear(c->acc, c->map, c->npix, c->d_stats, c->stream));
+    return DMANN_OK;
+}
+
+int dmann_get_stats(dmann_ctx *c, dmann_stats *st)
+{
+    if (!c || !st) return DMANN_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long h[kNumStats];
+    HIPCHK(c, hipMemcpy(h, c->d_stats, sizeof h, hipMemcpyDeviceToHost));
+    st->kept = (int64_t)h[kStatKept];
+    st->subpixel = (int64_t)h[kStatSub];
+    st->fallback = (int64_t)h[kStatFallback];
+    st->extended = (int64_t)h[kStatExt];
+    st->contributions = (int64_t)h[kStatContrib];
+    st->large = (int64_t)h[kStatLarge];
+    return DMANN_OK;
+}
+
+int64_t dmann_npix(const dmann_ctx *c) { return c ? c->npix : 0; }
+
+void dmann_destroy(dmann_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+const char *dmann_last_error(const dmann_ctx *c) { return c ? c->msg : g_create_msg; }
+
+static void fill_devparams(const dmann_params *p, DevParams &P)
+{
+    for (int i = 0; i < 3; i++) P.opos[i] = p->opos[i];
+    for (int i = 0; i < 9; i++) P.rot[i] = p->rotmatrix[i];
+    P.flux_factor = p->flux_factor;
+    P.mass_cut = p->mass_cut;
+    P.nside = p->nside;
+    P.npix = (int)npix_of(p->nside);
+    P.dOmega = 4 * kPi / (double)P.npix;
+    P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592));
+    P.large_pix = 1 << 16;
+}
+
+int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int32_t *pix, float *w, int64_t cap,
+                             double *flux, int *kind)
+{
+    if (!p || !q || p->nside < 1 || p->nside > kMaxNside) return -1;
+    DevParams P;
+    fill_devparams(p, P);
+    DiscRec rec;
+    bool bad = false;
+    const int cls = prepass_one(P, *reinterpret_cast<const Particle *>(q), rec, bad);
+    if (kind) *kind = 0;
+    if (bad) return -2;
+    if (cls == kClsSkip) return 0;
+    if (flux) *flux = rec.flux;
+    if (cls == kClsPoint) {
+        if (kind) *kind = 1;
+        if (cap > 0 && pix) pix[0] = rec.p0;
+        if (cap > 0 && w) w[0] = 1.0f;
+        return 1;
+    }
+    const PGeo g = particle_geo(rec, P.nside);
+    int64_t k = 0;
+    for (int gi = 0; gi < g.nrings; ++gi) {
+        const RingRec rr = ring_setup(g, P.nside, gi);
+        int abase, kw;
+        ring_addr(rr, (int)k, abase, kw);
+        for (int t = 0; t < rr.cnt; ++t, ++k) {
+            if (k < cap) {
+                if (pix) pix[k] = ring_pixel(abase, kw, rr.nr, (int)k);
+                if (w) w[k] = ring_weight32(g, rr, t);
+            }
+        }
+    }
+    if (kind) *kind = k < 2 ? 2 : 3;
+    if (k < 2) {
+        if (cap > 0 && pix) pix[0] = rec.p0;
+        if (cap > 0 && w) w[0] = 1.0f;
+    }
+    return k;
+}
+
+} // extern "C"

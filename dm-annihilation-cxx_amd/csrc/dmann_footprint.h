@@ -1,0 +1,242 @@
+// Per-particle and per-ring math of the projector, shared by the kernels and
+// by the host-side footprint enumerator (dmann_host_footprint), so the CPU
+// tests exercise exactly the code the GPU runs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dmann_geom.h"
+#include "dmann_kernels.h"
+
+namespace dmann {
+
+// Pre-pass of one particle (src/genmap.cpp:82-110; src/anglefuncs.cpp:15-71;
+// src/flux/flux_ann_nosom.cpp:12-23).  Returns kClsSkip (filtered),
+// kClsPoint (whole flux to rec.p0), kClsMedium or kClsLarge (footprint).
+// *bad is set when the direction is undefined (the reference exits there).
+DMANN_HD int prepass_one(const DevParams &P, const Particle &q, DiscRec &rec, bool &bad)
+{
+    bad = false;
+    if (((double)q.mass >= P.mass_cut) || (q.dens < 0)) return kClsSkip;
+    const double dx = (double)q.posx - P.opos[0];
+    const double dy = (double)q.posy - P.opos[1];
+    const double dz = (double)q.posz - P.opos[2];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const double flux = P.flux_factor * (double)q.dens * (double)q.mass / (4.0 * kPi * d * d);
+    const double v[3] = {dx, dy, dz};
+    double t[3];
+    for (int a = 0; a < 3; a++) {
+        t[a] = 0;
+        for (int b = 0; b < 3; b++) t[a] += P.rot[3 * b + a] * v[b];
+    }
+    const double costheta = t[2] / d;
+    double ph = atan2(t[1], t[0]);
+    if (ph < 0) ph += 2.0 * kPi;
+    ph -= kPi;
+    if (ph < 0) ph = 2.0 * kPi + ph;
+    if (ph > 2 * kPi) ph = ph - 2.0 * kPi;
+    const double theta = acos(costheta);
+    const double r = (double)q.hsmooth / d;
+    if (!(theta >= 0.0 && theta <= kPi)) {
+        bad = true;
+        return kClsSkip;
+    }
+    rec.theta = theta;
+    rec.phi = ph;
+    rec.r = r;
+    rec.flux = flux;
+    rec.p0 = ang2pix(P.nside, theta, ph);
+    rec.pad = 0;
+    if (2.0 * r < P.theta0) {
+        rec.est = 1;
+        rec.cls = kClsPoint;
+        return kClsPoint;
+    }
+    const double rad = 2.0 * r;
+    const double est = (rad >= kPi) ? (double)P.npix : kTwoPi * (1.0 - cos(rad)) / P.dOmega;
+    rec.est = est > 2.0e9 ? 2000000000 : (int)est;
+    rec.cls = (rad >= kPi || est > (double)P.large_pix) ? kClsLarge : kClsMedium;
+    return rec.cls;
+}
+
+// Per-particle quantities of the query_disc ring walk (FP64).
+struct PGeo {
+    double theta, phi, rad, r, z0, s0, xa, cosr, flux;
+    int irmin, irmax, nnorth, npart, nsouth, nrings, p0;
+    bool all;
+    int mode;  // 0: fast polynomials, 1: general (sin / asin)
+    float K4;  // -(0.5/0.333) * 4 * log2(e) / r^2
+};
+
+DMANN_HD PGeo particle_geo(const DiscRec &rec, int nside)
+{
+    PGeo g;
+    g.theta = rec.theta;
+    g.phi = fmodulo(rec.phi, kTwoPi); // pointing::normalize
+    g.r = rec.r;
+    g.rad = 2.0 * rec.r;
+    g.flux = rec.flux;
+    g.p0 = rec.p0;
+    g.all = g.rad >= kPi;
+    g.cosr = cos(g.rad);
+    g.z0 = cos(g.theta);
+    g.s0 = sqrt((1 - g.z0) * (1 + g.z0));
+    g.xa = 1. / g.s0;
+    const double rlat1 = g.theta - g.rad;
+    const double zmax = cos(rlat1);
+    g.irmin = ring_above(nside, zmax) + 1;
+    const bool north = (rlat1 <= 0) && (g.irmin > 1);
+    const double rlat2 = g.theta + g.rad;
+    const double zmin = cos(rlat2);
+    g.irmax = ring_above(nside, zmin);
+    const bool south = (rlat2 >= kPi) && (g.irmax + 1 < 4 * nside);
+    g.nnorth = north ? g.irmin - 1 : 0;
+    g.npart = g.irmax >= g.irmin ? g.irmax - g.irmin + 1 : 0;
+    g.nsouth = south ? 4 * nside - 1 - g.irmax : 0;
+    g.nrings = g.all ? 4 * nside - 1 : g.nnorth + g.npart + g.nsouth;
+    const double kc = 0.5 / 0.333; // src/genmap.cpp:143
+    g.K4 = (float)(-kc * 4.0 * 1.4426950408889634 / (g.r * g.r));
+    // the fast polynomials hold for psi <= 0.5 and |dphi| <= 0.5
+    bool fast = !g.all && !north && !south && g.rad <= 0.5;
+    if (fast) fast = sin(g.rad) < g.s0 * 0.43; // max |dphi| = asin(sin R / sin theta) < 0.445
+    g.mode = fast ? 0 : 1;
+    return g;
+}
+
+// One ring of a footprint.
+struct RingRec {
+    int cnt;
+    float A, B, offm, step; // haversine terms (FP32)
+    int kc_rel;             // column nearest phi, relative to col0
+    int col0, startpix, nr;
+    double z, s, shift;     // FP64 terms (FP64 weight form)
+};
+
+// Ring `gi` (0-based position in the particle's ring list: north polar cap
+// rings, then the partial rings irmin..irmax, then south polar cap rings).
+DMANN_HD RingRec ring_setup(const PGeo &g, int nside, int gi)
+{
+    RingRec o;
+    int iz;
+    bool full;
+    if (g.all) {
+        iz = gi + 1;
+        full = true;
+    } else if (gi < g.nnorth) {
+        iz = gi + 1;
+        full = true;
+    } else if (gi < g.nnorth + g.npart) {
+        iz = g.irmin + (gi - g.nnorth);
+        full = false;
+    } else {
+        iz = g.irmax + 1 + (gi - g.nnorth - g.npart);
+        full = true;
+    }
+    int startpix, nr, shifted;
+    ring_info(nside, iz, startpix, nr, shifted);
+    const double shift = shifted ? 0.5 : 0.0;
+    const double z = ring2z(nside, iz);
+    const double fnr = (double)nr;
+    const int kc = (int)floor(fnr * kInvTwoPi * g.phi - shift + 0.5);
+    int col0 = 0, cnt = 0;
+    if (full) {
+        col0 = kc - nr / 2;
+        cnt = nr;
+    } else {
+        // Healpix_Base::query_disc (non-inclusive), called at src/genmap.cpp:119
+        const double x = (g.cosr - z * g.z0) * g.xa;
+        const double ysq = 1 - z * z - x * x;
+        const double dphi = (ysq <= 0) ? 0 : atan2(sqrt(ysq), x);
+        if (dphi > 0) {
+            const int ip_lo = (int)floor(fnr * kInvTwoPi * (g.phi - dphi) - shift) + 1;
+            const int ip_hi = (int)floor(fnr * kInvTwoPi * (g.phi + dphi) - shift);
+            if (ip_lo <= ip_hi) {
+                col0 = ip_lo;
+                cnt = ip_hi - ip_lo + 1;
+            }
+        }
+    }
+    const double s = sqrt((1 - z) * (1 + z));
+    double A = (1.0 - (z * g.z0 + s * g.s0)) * 0.5; // hav(theta_ring - theta)
+    if (A < 0) A = 0;
+    const double step = kTwoPi / fnr;
+    o.cnt = cnt;
+    o.A = (float)A;
+    o.B = (float)(s * g.s0);
+    o.offm = (float)(((double)kc + shift) * step - g.phi);
+    o.step = (float)step;
+    o.kc_rel = kc - col0;
+    o.col0 = col0;
+    o.startpix = startpix;
+    o.nr = nr;
+    o.z = z;
+    o.s = s;
+    o.shift = shift;
+    return o;
+}
+
+// Address parameters of a ring whose first footprint pixel has flat index
+// pre_start: pixel(k) = k + abase - (k >= kwrap ? nr : 0).  Columns run
+// col0 .. col0+cnt-1 unwrapped, with col0 in [-nr/2-1, nr) and cnt <= nr.
+DMANN_HD void ring_addr(const RingRec &rr, int pre_start, int &abase, int &kwrap)
+{
+    abase = rr.startpix + rr.col0 - pre_start;
+    if (rr.col0 < 0) {
+        abase += rr.nr;
+        kwrap = pre_start - rr.col0;
+    } else {
+        kwrap = pre_start + rr.nr - rr.col0;
+    }
+}
+
+DMANN_HD int ring_pixel(int abase, int kwrap, int nr, int k) { return k + abase - (k >= kwrap ? nr : 0); }
+
+DMANN_HD float fast_exp2(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_exp2f(x);
+#else
+    return exp2f(x);
+#endif
+}
+
+// Gaussian weight of a pixel at longitude offset x from the particle in a
+// ring with haversine terms (A, B): h = hav(psi) = A + B sin^2(x/2),
+// psi/2 = asin(sqrt h), weight = exp(-0.5 (psi/r)^2 / 0.333)
+// (src/genmap.cpp:137-143) = exp2(K4 asin^2(sqrt h)).
+template <int MODE>
+DMANN_HD float weight32(float x, float A, float B, float K4)
+{
+    if (MODE == 0) {
+        // sin^2(x/2) = u (1/4 - u/48 + u^2/1440 - u^3/80640), u = x^2, |x| <= 0.5
+        const float u = x * x;
+        float p = fmaf(u, -1.0f / 80640.0f, 1.0f / 1440.0f);
+        p = fmaf(u, p, -1.0f / 48.0f);
+        p = fmaf(u, p, 0.25f);
+        const float hphi = u * p;
+        const float h = fmaf(B, hphi, A);
+        // asin^2(sqrt h)/h = sum_n 2^(2n) h^(n-1) / (2 n^2 C(2n,n)), h <= 0.0612
+        float gq = fmaf(h, 128.0f / 2079.0f, 128.0f / 1575.0f);
+        gq = fmaf(h, gq, 4.0f / 35.0f);
+        gq = fmaf(h, gq, 8.0f / 45.0f);
+        gq = fmaf(h, gq, 1.0f / 3.0f);
+        gq = fmaf(h, gq, 1.0f);
+        return fast_exp2(K4 * (h * gq));
+    } else {
+        const float sh = sinf(0.5f * x);
+        float h = fmaf(B, sh * sh, A);
+        h = fminf(h, 1.0f);
+        const float a = asinf(sqrtf(h));
+        return fast_exp2(K4 * (a * a));
+    }
+}
+
+DMANN_HD float ring_weight32(const PGeo &g, const RingRec &rr, int t)
+{
+    const float kf = (float)(t - rr.kc_rel);
+    const float x = fmaf(kf, rr.step, rr.offm);
+    return g.mode == 0 ? weight32<0>(x, rr.A, rr.B, g.K4) : weight32<1>(x, rr.A, rr.B, g.K4);
+}
+
+} // namespace dmann

@@ -1,0 +1,74 @@
+// Internal interface between the C ABI (dmann_capi.hip) and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dmann {
+
+// 48-byte particle record (src/datatypes.h:46-105).
+struct Particle {
+    float mass, dens, hsmooth;
+    float posx, posy, posz;
+    float velx, vely, velz;
+    float sigmav, eps, phi;
+};
+static_assert(sizeof(Particle) == 48, "DMParticle is 12 floats");
+
+enum { kAccF32 = 0, kAccComp = 1, kAccF64 = 2 };
+enum { kClsSkip = 0, kClsPoint = 1, kClsMedium = 2, kClsLarge = 3 };
+enum {
+    kStatKept = 0,
+    kStatSub = 1,
+    kStatFallback = 2,
+    kStatExt = 3,
+    kStatContrib = 4,
+    kStatLarge = 5,
+    kStatBad = 6,
+    kNumStats = 8
+};
+
+// Record of one extended particle after the pre-pass (48 bytes).
+struct DiscRec {
+    double theta, phi, r, flux;
+    int p0;  // RING pixel of the direction (fallback deposit)
+    int est; // estimated footprint pixels
+    int cls;
+    int pad;
+};
+static_assert(sizeof(DiscRec) == 48, "DiscRec is 48 bytes");
+
+// Per-map constants (host-computed).
+struct DevParams {
+    double opos[3];
+    double rot[9];
+    double flux_factor;
+    double mass_cut;
+    double theta0;  // acos(1 - dOmega / (2 * 3.141592)), src/genmap.cpp:38
+    double dOmega;
+    int nside;
+    int npix;
+    int large_pix;  // estimated footprint above which the sky-covering path is used
+    int pad;
+};
+
+// Accumulation targets of one map.
+struct MapRef {
+    float *f32;  // FP32 map, or the hi word of the two-float map
+    float *lo;   // lo word of the two-float map
+    double *f64; // FP64 map
+};
+
+struct LaunchCfg {
+    int prepass_blocks;
+    int medium_blocks;
+    int large_blocks;
+};
+
+hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
+                         unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
+                         hipStream_t st);
+hipError_t launch_finalize(int acc, MapRef map, int64_t npix, double dOmega, double *out, hipStream_t st);
+hipError_t launch_clear(int acc, MapRef map, int64_t npix, unsigned long long *stats, hipStream_t st);
+
+} // namespace dmann

@@ -1,0 +1,520 @@
+// HIP kernels of the MI355X annihilation-map projector (gfx950, wave64).
+//
+// Pipeline for one render call (all on the context's stream):
+//   k_prepass      one thread per particle, FP64: filter, observer-frame
+//                  distance, flux, angles, angular radius, RING pixel; the
+//                  sub-pixel rule deposits whole particles; extended
+//                  particles get a 48-byte record in a medium or a
+//                  sky-covering (large) class list.
+//   k_disc_medium  one wavefront per particle: FP64 query_disc ring ranges
+//                  (one ring per lane) staged in LDS, Gaussian weights in
+//                  FP32 from a haversine form, normalisation, deposit.
+//   k_disc_large   one 256-thread workgroup per sky-covering particle,
+//                  ring-major, FP64 normalisation sum.
+//   k_finalize     map / dOmega into the FP64 RING output.
+//
+// Reference semantics followed: src/genmap.cpp:22-175 (CPU generator).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dmann_footprint.h"
+#include "dmann_geom.h"
+#include "dmann_kernels.h"
+
+namespace dmann {
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Adds one value to the map in the context's accumulation form.
+template <int ACC>
+__device__ __forceinline__ void acc_add(const MapRef &m, int addr, float v32, double v64)
+{
+    if constexpr (ACC == kAccF32) {
+        unsafeAtomicAdd(m.f32 + addr, v32);
+    } else if constexpr (ACC == kAccComp) {
+        // two-float accumulation: hi by a returning atomic, then the exact
+        // rounding error of that addition (TwoSum) into lo.
+        float old = unsafeAtomicAdd(m.f32 + addr, v32);
+        float s = old + v32;
+        float bb = s - old;
+        float err = (old - (s - bb)) + (v32 - bb);
+        if (err != 0.0f) unsafeAtomicAdd(m.lo + addr, err);
+    } else {
+        unsafeAtomicAdd(m.f64 + addr, v64);
+    }
+}
+
+// Whole-particle deposit (sub-pixel rule and the npix_disc<2 fallback,
+// src/genmap.cpp:107-110,125-128).
+template <int ACC>
+__device__ __forceinline__ void acc_point(const MapRef &m, int addr, double flux)
+{
+    if constexpr (ACC == kAccComp) {
+        float f = (float)flux;
+        float rem = (float)(flux - (double)f);
+        float old = unsafeAtomicAdd(m.f32 + addr, f);
+        float s = old + f;
+        float bb = s - old;
+        float err = (old - (s - bb)) + (f - bb) + rem;
+        if (err != 0.0f) unsafeAtomicAdd(m.lo + addr, err);
+    } else {
+        acc_add<ACC>(m, addr, (float)flux, flux);
+    }
+}
+
+// Wave-aggregated append: every lane of the wave must call it.
+__device__ __forceinline__ unsigned wave_append(unsigned *counter, bool pred)
+{
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const unsigned rank = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
+
+__device__ __forceinline__ void wave_count(unsigned long long *ctr, unsigned long long v)
+{
+    // v is per lane; one atomic per wave
+    unsigned long long s = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (__lane_id() == 0 && s) atomicAdd(ctr, s);
+}
+
+// ---------------------------------------------------------------- pre-pass
+
+// One thread per particle (grid-stride).  Mirrors src/genmap.cpp:82-110 with
+// the helpers of src/anglefuncs.cpp:15-71 and src/flux/flux_ann_nosom.cpp:12-23.
+template <int ACC>
+__global__ __launch_bounds__(256) void k_prepass(const Particle *__restrict__ parts, int64_t n,
+                                                 DevParams P, DiscRec *__restrict__ recs,
+                                                 unsigned *__restrict__ counts, MapRef map,
+                                                 unsigned long long *__restrict__ stats)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long n_kept = 0, n_sub = 0, n_bad = 0;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int64_t i = base + threadIdx.x;
+        int cls = kClsSkip;
+        DiscRec rec;
+        if (i < n) {
+            const Particle q = parts[i];
+            bool bad;
+            const int c = prepass_one(P, q, rec, bad);
+            if (bad) n_bad++;
+            if (c != kClsSkip) n_kept++;
+            if (c == kClsPoint) {
+                n_sub++;
+                acc_point<ACC>(map, rec.p0, rec.flux);
+            } else if (c == kClsMedium || c == kClsLarge) {
+                cls = c;
+            }
+        }
+        const unsigned im = wave_append(&counts[0], cls == kClsMedium);
+        const unsigned il = wave_append(&counts[1], cls == kClsLarge);
+        if (cls == kClsMedium) recs[im] = rec;
+        if (cls == kClsLarge) recs[n - 1 - (int64_t)il] = rec;
+    }
+    wave_count(&stats[kStatKept], n_kept);
+    wave_count(&stats[kStatSub], n_sub);
+    wave_count(&stats[kStatBad], n_bad);
+}
+
+// ---------------------------------------------------------------- footprint
+
+// FP64 weight exactly as the reference forms it (src/genmap.cpp:133-143):
+// centre vector, acos of the dot product, exp(-0.5 d2 / 0.333).
+__device__ __forceinline__ double weight64(double z, double s, double phic, const PGeo &g,
+                                           double vx, double vy)
+{
+    const double ux = s * cos(phic), uy = s * sin(phic);
+    double d2 = acos(ux * vx + uy * vy + z * g.z0) / g.r;
+    d2 = d2 * d2;
+    return exp(-0.5 * d2 / 0.333);
+}
+
+// LDS layout of one ring window (64 rings) for one wave.
+struct WinLDS {
+    int pre_end[64];
+    int kref[64];     // pre_start + kc_rel
+    float4 f[64];     // A, B, offm, step
+    int4 a[64];       // abase, k_wrap, nr, kcol (= pre_start - col0)
+    double4 d[64];    // z, s, shift, 2pi/nr  (FP64 form only)
+};
+
+// Sets up rings [win*64, win*64+64) of the particle; returns the window's
+// pixel count.  Lane j handles ring win*64+j.
+template <bool F64>
+__device__ __forceinline__ int setup_window(const PGeo &g, int nside, int win, WinLDS &L, int lane,
+                                            int kbase)
+{
+    const int gi = win * 64 + lane;
+    RingRec rr;
+    int cnt = 0;
+    if (gi < g.nrings) {
+        rr = ring_setup(g, nside, gi);
+        cnt = rr.cnt;
+    }
+    const int incl = wave_incl_scan(cnt, lane);
+    const int total = __shfl(incl, 63, 64);
+    const int pre_start = kbase + incl - cnt;
+    L.pre_end[lane] = kbase + incl;
+    if (gi < g.nrings) {
+        L.kref[lane] = pre_start + rr.kc_rel;
+        L.f[lane] = make_float4(rr.A, rr.B, rr.offm, rr.step);
+        int abase, kw;
+        ring_addr(rr, pre_start, abase, kw);
+        L.a[lane] = make_int4(abase, kw, rr.nr, pre_start - rr.col0);
+        if constexpr (F64) L.d[lane] = make_double4(rr.z, rr.s, rr.shift, kTwoPi / (double)rr.nr);
+    }
+    wave_sync();
+    return total;
+}
+
+__device__ __forceinline__ int pix_addr(const int4 a, int k) { return ring_pixel(a.x, a.y, a.z, k); }
+
+// Register capacity of the single-evaluation path: footprints of up to
+// 64*kNW pixels keep their weights in VGPRs between the norm and the deposit.
+constexpr int kNW = 24;
+
+// Renders one particle with one wavefront.  Returns the footprint size.
+template <int ACC, int MODE>
+__device__ int render_particle_wave(const PGeo &g, int nside, WinLDS &L, int lane, const MapRef &map)
+{
+    const int nwin = (g.nrings + 63) / 64;
+    if constexpr (ACC != kAccF64) {
+        if (nwin == 1) {
+            const int T = setup_window<false>(g, nside, 0, L, lane, 0);
+            if (T < 2) {
+                if (lane == 0) acc_point<ACC>(map, g.p0, g.flux);
+                wave_sync();
+                return T;
+            }
+            if (T <= 64 * kNW) {
+                float wv[kNW];
+                int av[kNW];
+                float nsum = 0.0f;
+                int j = 0;
+#pragma unroll
+                for (int it = 0; it < kNW; ++it) {
+                    const int k = lane + 64 * it;
+                    wv[it] = 0.0f;
+                    av[it] = 0;
+                    if (k < T) {
+                        while (L.pre_end[j] <= k) ++j;
+                        const float4 f = L.f[j];
+                        const float kf = (float)(k - L.kref[j]);
+                        const float w = weight32<MODE>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                        wv[it] = w;
+                        av[it] = pix_addr(L.a[j], k);
+                        nsum += w;
+                    }
+                }
+                const double norm = wave_sum((double)nsum);
+                const float scale = (float)(g.flux / norm);
+#pragma unroll
+                for (int it = 0; it < kNW; ++it) {
+                    const int k = lane + 64 * it;
+                    if (k < T) acc_add<ACC>(map, av[it], wv[it] * scale, 0.0);
+                }
+                wave_sync();
+                return T;
+            }
+            wave_sync();
+        }
+    }
+    // two passes over ring windows: normalisation, then deposit
+    double nacc = 0.0;
+    int T = 0;
+    const double vx = sin(g.theta) * cos(g.phi), vy = sin(g.theta) * sin(g.phi);
+    for (int win = 0; win < nwin; ++win) {
+        const int Tw = setup_window<ACC == kAccF64>(g, nside, win, L, lane, 0);
+        int j = 0;
+        if constexpr (ACC == kAccF64) {
+            for (int k = lane; k < Tw; k += 64) {
+                while (L.pre_end[j] <= k) ++j;
+                const double4 dd = L.d[j];
+                const int col = k - L.a[j].w;
+                nacc += weight64(dd.x, dd.y, ((double)col + dd.z) * dd.w, g, vx, vy);
+            }
+        } else {
+            float part = 0.0f;
+            int c = 0;
+            for (int k = lane; k < Tw; k += 64) {
+                while (L.pre_end[j] <= k) ++j;
+                const float4 f = L.f[j];
+                const float kf = (float)(k - L.kref[j]);
+                part += weight32<MODE>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                if (++c == 16) {
+                    nacc += (double)part;
+                    part = 0.0f;
+                    c = 0;
+                }
+            }
+            nacc += (double)part;
+        }
+        T += Tw;
+        wave_sync();
+    }
+    if (T < 2) {
+        if (lane == 0) acc_point<ACC>(map, g.p0, g.flux);
+        wave_sync();
+        return T;
+    }
+    const double norm = wave_sum(nacc);
+    const double scale64 = g.flux / norm;
+    const float scale = (float)scale64;
+    for (int win = 0; win < nwin; ++win) {
+        const int Tw = setup_window<ACC == kAccF64>(g, nside, win, L, lane, 0);
+        int j = 0;
+        for (int k = lane; k < Tw; k += 64) {
+            while (L.pre_end[j] <= k) ++j;
+            const int addr = pix_addr(L.a[j], k);
+            if constexpr (ACC == kAccF64) {
+                const double4 dd = L.d[j];
+                const int col = k - L.a[j].w;
+                const double w = weight64(dd.x, dd.y, ((double)col + dd.z) * dd.w, g, vx, vy);
+                acc_add<ACC>(map, addr, 0.0f, (w / norm) * g.flux);
+            } else {
+                const float4 f = L.f[j];
+                const float kf = (float)(k - L.kref[j]);
+                const float w = weight32<MODE>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                acc_add<ACC>(map, addr, w * scale, 0.0);
+            }
+        }
+        wave_sync();
+    }
+    (void)scale64;
+    return T;
+}
+
+// One wavefront per particle of the medium class (grid-stride over waves).
+template <int ACC>
+__global__ __launch_bounds__(256) void k_disc_medium(const DiscRec *__restrict__ recs,
+                                                     const unsigned *__restrict__ counts, int nside,
+                                                     MapRef map, unsigned long long *__restrict__ stats)
+{
+    __shared__ WinLDS lds[4];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    WinLDS &L = lds[wid];
+    const unsigned count = counts[0];
+    const unsigned nwaves = gridDim.x * 4;
+    unsigned long long n_ext = 0, n_fb = 0, n_contrib = 0;
+    for (unsigned it = blockIdx.x * 4 + wid; it < count; it += nwaves) {
+        const PGeo g = particle_geo(recs[it], nside);
+        int T;
+        if (g.mode == 0) T = render_particle_wave<ACC, 0>(g, nside, L, lane, map);
+        else T = render_particle_wave<ACC, 1>(g, nside, L, lane, map);
+        if (T < 2) n_fb++;
+        else {
+            n_ext++;
+            n_contrib += (unsigned long long)T;
+        }
+    }
+    if (lane == 0) {
+        if (n_ext) atomicAdd(&stats[kStatExt], n_ext);
+        if (n_fb) atomicAdd(&stats[kStatFallback], n_fb);
+        if (n_contrib) atomicAdd(&stats[kStatContrib], n_contrib);
+    }
+}
+
+// Sky-covering particles: one 256-thread workgroup per particle; each wave
+// walks whole rings (ring-major) from 64-ring windows, FP64 normalisation.
+template <int ACC>
+__global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ recs, int64_t n,
+                                                    const unsigned *__restrict__ counts, int nside,
+                                                    MapRef map, unsigned long long *__restrict__ stats)
+{
+    __shared__ WinLDS lds[4];
+    __shared__ double s_red[4];
+    __shared__ int s_cnt[4];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    WinLDS &L = lds[wid];
+    const unsigned count = counts[1];
+    for (unsigned it = blockIdx.x; it < count; it += gridDim.x) {
+        const PGeo g = particle_geo(recs[n - 1 - (int64_t)it], nside);
+        const int nwin = (g.nrings + 63) / 64;
+        const double vx = sin(g.theta) * cos(g.phi), vy = sin(g.theta) * sin(g.phi);
+        double scale64 = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {
+            double nacc = 0.0;
+            int T = 0;
+            for (int win = wid; win < nwin; win += 4) {
+                const int Tw = setup_window<ACC == kAccF64>(g, nside, win, L, lane, 0);
+                T += Tw;
+                for (int j = 0; j < 64; ++j) {
+                    const int pre_start = j ? L.pre_end[j - 1] : 0;
+                    const int cnt = L.pre_end[j] - pre_start;
+                    if (cnt <= 0) continue;
+                    const float4 f = L.f[j];
+                    const int kref = L.kref[j];
+                    const int4 a = L.a[j];
+                    const double4 dd = L.d[j];
+                    float part = 0.0f;
+                    for (int t = lane; t < cnt; t += 64) {
+                        const int k = pre_start + t;
+                        if constexpr (ACC == kAccF64) {
+                            const double w = weight64(dd.x, dd.y, ((double)(k - a.w) + dd.z) * dd.w, g, vx, vy);
+                            if (pass == 0) nacc += w;
+                            else acc_add<ACC>(map, pix_addr(a, k), 0.0f, (w * scale64));
+                        } else {
+                            const float kf = (float)(k - kref);
+                            float w;
+                            if (g.mode == 0) w = weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                            else w = weight32<1>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                            if (pass == 0) part += w;
+                            else acc_add<ACC>(map, pix_addr(a, k), w * (float)scale64, 0.0);
+                        }
+                    }
+                    if constexpr (ACC != kAccF64) nacc += (double)part;
+                }
+                wave_sync();
+            }
+            if (pass == 0) {
+                const double ws = wave_sum(nacc);
+                if (lane == 0) {
+                    s_red[wid] = ws;
+                    s_cnt[wid] = T;
+                }
+                __syncthreads();
+                const double norm = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+                const int Ttot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+                __syncthreads();
+                if (Ttot < 2) {
+                    if (threadIdx.x == 0) {
+                        acc_point<ACC>(map, g.p0, g.flux);
+                        atomicAdd(&stats[kStatFallback], 1ull);
+                    }
+                    break;
+                }
+                if (ACC == kAccF64) scale64 = g.flux / norm;
+                else scale64 = g.flux / norm;
+                if (threadIdx.x == 0) {
+                    atomicAdd(&stats[kStatExt], 1ull);
+                    atomicAdd(&stats[kStatLarge], 1ull);
+                    atomicAdd(&stats[kStatContrib], (unsigned long long)Ttot);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- map ops
+
+__global__ void k_fill_f32(float *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+__global__ void k_fill_f64(double *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+__global__ void k_fill_u32(unsigned *p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+// map / dOmega into FP64 RING output (src/genmap.cpp:164-169).
+template <int ACC>
+__global__ void k_finalize(MapRef map, int64_t npix, double dOmega, double *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v;
+        if constexpr (ACC == kAccF32) v = (double)map.f32[i];
+        else if constexpr (ACC == kAccComp) v = (double)map.f32[i] + (double)map.lo[i];
+        else v = map.f64[i];
+        out[i] = v / dOmega;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <int ACC>
+static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
+                                  unsigned *counts, MapRef map, unsigned long long *stats,
+                                  const LaunchCfg &cfg, hipStream_t st)
+{
+    k_fill_u32<<<1, 64, 0, st>>>(counts, 2);
+    const int64_t want = (n + 255) / 256;
+    const int pre_blocks = (int)(want < cfg.prepass_blocks ? want : cfg.prepass_blocks);
+    if (pre_blocks > 0)
+        k_prepass<ACC><<<pre_blocks, 256, 0, st>>>(d_parts, n, P, recs, counts, map, stats);
+    k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, map, stats);
+    k_disc_medium<ACC><<<cfg.medium_blocks, 256, 0, st>>>(recs, counts, P.nside, map, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
+                         unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
+                         hipStream_t st)
+{
+    switch (acc) {
+    case kAccF32: return launch_render_t<kAccF32>(d_parts, n, P, recs, counts, map, stats, cfg, st);
+    case kAccComp: return launch_render_t<kAccComp>(d_parts, n, P, recs, counts, map, stats, cfg, st);
+    default: return launch_render_t<kAccF64>(d_parts, n, P, recs, counts, map, stats, cfg, st);
+    }
+}
+
+hipError_t launch_finalize(int acc, MapRef map, int64_t npix, double dOmega, double *out, hipStream_t st)
+{
+    const int blocks = (int)((npix + 255) / 256 < 8192 ? (npix + 255) / 256 : 8192);
+    switch (acc) {
+    case kAccF32: k_finalize<kAccF32><<<blocks, 256, 0, st>>>(map, npix, dOmega, out); break;
+    case kAccComp: k_finalize<kAccComp><<<blocks, 256, 0, st>>>(map, npix, dOmega, out); break;
+    default: k_finalize<kAccF64><<<blocks, 256, 0, st>>>(map, npix, dOmega, out); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_clear(int acc, MapRef map, int64_t npix, unsigned long long *stats, hipStream_t st)
+{
+    const int blocks = (int)((npix + 255) / 256 < 8192 ? (npix + 255) / 256 : 8192);
+    if (acc == kAccF64) k_fill_f64<<<blocks, 256, 0, st>>>(map.f64, npix);
+    else k_fill_f32<<<blocks, 256, 0, st>>>(map.f32, npix);
+    if (acc == kAccComp) k_fill_f32<<<blocks, 256, 0, st>>>(map.lo, npix);
+    k_fill_u64<<<1, 64, 0, st>>>(stats, kNumStats);
+    return hipGetLastError();
+}
+
+} // namespace dmann

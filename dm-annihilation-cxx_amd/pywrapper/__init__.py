@@ -1,0 +1,11 @@
+"""Python front end of the MI355X annihilation-map projector.
+
+The reference's pywrapper (src/pywrapper/) only exposed a particle reader;
+this package exposes the map path itself:
+  capi      ctypes binding of libdmann.so (Projector, params, FITS I/O)
+  synth     seeded synthetic NFW / Plummer particle sets
+"""
+from . import capi, synth  # noqa: F401
+from .capi import (PARTICLE_DTYPE, PREC_COMPENSATED, PREC_FP32, PREC_FP64, DmannError,  # noqa: F401
+                   DmannParams, Projector, load_library, params_for_observer, params_from_ini,
+                   read_fits, write_fits, write_ini)

@@ -1,0 +1,277 @@
+"""ctypes binding of libdmann.so (include/dmann.h).
+
+The library is built in-tree by `make -C dm-annihilation-cxx_amd`; importing
+this module fails loudly when it is missing: there is no CPU fallback.
+Device-resident inputs are torch tensors on a ROCm device; the context then
+runs on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libdmann.so")
+
+PREC_FP32, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2
+PRECISIONS = {"fp32": PREC_FP32, "compensated": PREC_COMPENSATED, "fp64": PREC_FP64}
+
+# 48-byte DMParticle record (src/datatypes.h:46-105)
+PARTICLE_FIELDS = ("mass", "dens", "hsmooth", "posx", "posy", "posz",
+                   "velx", "vely", "velz", "sigmav", "eps", "phi")
+PARTICLE_DTYPE = np.dtype([(n, "<f4") for n in PARTICLE_FIELDS])
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+class DmannParams(C.Structure):
+    _fields_ = [("opos", C.c_double * 3), ("rotmatrix", C.c_double * 9),
+                ("flux_factor", C.c_double), ("mass_cut", C.c_double),
+                ("nside", C.c_int32), ("precision", C.c_int32)]
+
+    def copy(self) -> "DmannParams":
+        p = DmannParams()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(DmannParams))
+        return p
+
+
+class DmannStats(C.Structure):
+    _fields_ = [("kept", C.c_int64), ("subpixel", C.c_int64), ("fallback", C.c_int64),
+                ("extended", C.c_int64), ("contributions", C.c_int64), ("large", C.c_int64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_render_device",
+           "dmann_set_stream", "dmann_synchronize", "dmann_get_map", "dmann_get_map_device",
+           "dmann_reset", "dmann_get_stats", "dmann_npix", "dmann_destroy",
+           "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint",
+           "dmann_last_error")
+
+_lib = None
+
+
+class DmannError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dmann status {code}: {msg}")
+        self.code = code
+
+
+def load_library():
+    """Loads libdmann.so with its prototypes (no GPU call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `make -C {PKG_DIR}`")
+    L = C.CDLL(LIB_PATH)
+    vp, i64 = C.c_void_p, C.c_int64
+    L.dmann_params_from_ini.argtypes = [C.c_char_p, C.c_int, C.POINTER(DmannParams),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.dmann_create.argtypes = [C.POINTER(vp), C.POINTER(DmannParams), i64, C.c_int]
+    L.dmann_render.argtypes = [vp, vp, i64]
+    L.dmann_render_device.argtypes = [vp, vp, i64]
+    L.dmann_set_stream.argtypes = [vp, vp]
+    L.dmann_synchronize.argtypes = [vp]
+    L.dmann_get_map.argtypes = [vp, vp]
+    L.dmann_get_map_device.argtypes = [vp, vp]
+    L.dmann_reset.argtypes = [vp]
+    L.dmann_get_stats.argtypes = [vp, C.POINTER(DmannStats)]
+    L.dmann_npix.argtypes = [vp]
+    L.dmann_npix.restype = i64
+    L.dmann_destroy.argtypes = [vp]
+    L.dmann_destroy.restype = None
+    L.dmann_write_healpix_fits.argtypes = [C.c_char_p, vp, i64, C.c_int]
+    L.dmann_read_healpix_fits.argtypes = [C.c_char_p, vp, i64, C.POINTER(C.c_int), C.POINTER(i64)]
+    L.dmann_host_footprint.argtypes = [C.POINTER(DmannParams), vp, vp, vp, i64, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int)]
+    L.dmann_host_footprint.restype = i64
+    L.dmann_last_error.argtypes = [vp]
+    L.dmann_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(code, ctx=None):
+    if code != 0:
+        msg = load_library().dmann_last_error(ctx)
+        raise DmannError(code, (msg or b"").decode(errors="replace"))
+
+
+# ------------------------------------------------------------------ params
+
+def params_from_ini(path: str, precision="compensated"):
+    """(DmannParams, mem_buffer, test) from a reference ini file."""
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    p = DmannParams()
+    mb, tn = C.c_int(), C.c_int()
+    _check(load_library().dmann_params_from_ini(path.encode(), prec, C.byref(p), C.byref(mb), C.byref(tn)))
+    return p, mb.value, tn.value
+
+
+def write_ini(path: str, *, nside: int, obs_pos=(8.0, 0.0, 0.0), centerpos=(0.0, 0.0, 0.0),
+              align_vec=(0.0, 0.0, 1.0), particle_masses=(4100.0,), Omega_M=0.238, h100=0.73,
+              Lbox_in_Mpc=0.001, mass_to_Msun=1.0, mem_buffer=10000000, datafile="particles.bin",
+              outputmap="skymap.fits", extra: dict | None = None) -> str:
+    """Writes an ini for the synthetic workloads: code length 1 kpc, code
+    mass 1 Msun, density in units of the critical density."""
+    f3 = lambda v: " ".join(repr(float(x)) for x in v)
+    lines = [f"Omega_M = {Omega_M}", f"h100 = {h100}", f"Lbox_in_Mpc = {Lbox_in_Mpc}",
+             f"mass_to_Msun = {mass_to_Msun}",
+             "particle_masses = " + " ".join(repr(float(m)) for m in particle_masses),
+             f"centerpos = {f3(centerpos)}", f"align_vec = {f3(align_vec)}", f"obs_pos = {f3(obs_pos)}",
+             f"nside = {int(nside)}", f"mem_buffer = {int(mem_buffer)}", f"datafile = {datafile}",
+             f"outputmap = {outputmap}"]
+    for k, v in (extra or {}).items():
+        lines.append(f"{k} = {v}")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
+
+
+def params_for_observer(nside: int, precision="compensated", **ini_kw) -> DmannParams:
+    """Map constants for an observer, through the library's own Parameters."""
+    with tempfile.TemporaryDirectory() as d:
+        path = write_ini(os.path.join(d, "obs.ini"), nside=nside, **ini_kw)
+        p, _, _ = params_from_ini(path, precision)
+    return p
+
+
+# ------------------------------------------------------------------ context
+
+def _as_host_records(parts) -> np.ndarray:
+    a = np.ascontiguousarray(parts)
+    if a.dtype == PARTICLE_DTYPE:
+        return a
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 12:
+        raise ValueError("particles must be a DMParticle record array or an (n, 12) float32 array")
+    return a
+
+
+class Projector:
+    """One map on one device (a dmann_ctx)."""
+
+    def __init__(self, params: DmannParams, max_parts: int, device: int = 0):
+        L = load_library()
+        self._L = L
+        self.params = params.copy()
+        h = C.c_void_p()
+        _check(L.dmann_create(C.byref(h), C.byref(self.params), int(max_parts), int(device)))
+        self._h = h
+        self.device = device
+        self.npix = int(L.dmann_npix(h))
+        self.max_parts = int(max_parts)
+
+    # -- rendering
+    def render(self, parts) -> None:
+        """Adds particles (host numpy records, or a torch (n, 12) float32
+        tensor on the context's device) to the map."""
+        try:
+            import torch
+            is_tensor = isinstance(parts, torch.Tensor)
+        except ImportError:  # pragma: no cover
+            is_tensor = False
+        if is_tensor:
+            t = parts
+            if t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 12 or not t.is_contiguous():
+                raise ValueError("device particles must be a contiguous (n, 12) float32 tensor")
+            if not t.is_cuda:
+                self.render(t.numpy())
+                return
+            self._bind_stream()
+            _check(self._L.dmann_render_device(self._h, C.c_void_p(t.data_ptr()), int(t.shape[0])), self._h)
+            return
+        a = _as_host_records(parts)
+        n = a.shape[0]
+        _check(self._L.dmann_render(self._h, C.c_void_p(a.ctypes.data), int(n)), self._h)
+
+    def _bind_stream(self):
+        import torch
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        _check(self._L.dmann_set_stream(self._h, C.c_void_p(s)), self._h)
+
+    def synchronize(self) -> None:
+        _check(self._L.dmann_synchronize(self._h), self._h)
+
+    def get_map(self) -> np.ndarray:
+        out = np.empty(self.npix, dtype=np.float64)
+        _check(self._L.dmann_get_map(self._h, C.c_void_p(out.ctypes.data)), self._h)
+        return out
+
+    def get_map_device(self, out):
+        """Writes the finished map into a float64 CUDA tensor of npix."""
+        import torch
+        if out.dtype != torch.float64 or out.numel() != self.npix or not out.is_cuda:
+            raise ValueError("out must be a float64 device tensor of npix elements")
+        self._bind_stream()
+        _check(self._L.dmann_get_map_device(self._h, C.c_void_p(out.data_ptr())), self._h)
+        return out
+
+    def reset(self) -> None:
+        _check(self._L.dmann_reset(self._h), self._h)
+
+    def stats(self) -> dict:
+        st = DmannStats()
+        _check(self._L.dmann_get_stats(self._h, C.byref(st)), self._h)
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.dmann_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def host_footprint(params: DmannParams, rec):
+    """(kind, pixels, FP32 weights, flux) of one particle, computed on the
+    host by the kernels' own per-particle and per-ring code."""
+    L = load_library()
+    r = np.ascontiguousarray(np.asarray(rec).reshape(-1)[:1]).view(np.float32)
+    if r.size != 12:
+        r = np.ascontiguousarray(np.asarray(rec, dtype=np.float32).reshape(12))
+    cap = 1 << 12
+    while True:
+        pix = np.zeros(cap, dtype=np.int32)
+        w = np.zeros(cap, dtype=np.float32)
+        fl, kind = C.c_double(), C.c_int()
+        n = L.dmann_host_footprint(C.byref(params), C.c_void_p(r.ctypes.data), C.c_void_p(pix.ctypes.data),
+                                   C.c_void_p(w.ctypes.data), cap, C.byref(fl), C.byref(kind))
+        if n < 0:
+            raise DmannError(int(n), "bad particle")
+        if n <= cap:
+            m = 1 if kind.value in (1, 2) else int(n)
+            return kind.value, pix[:m].copy(), w[:m].copy(), fl.value
+        cap = int(n)
+
+
+# ------------------------------------------------------------------ FITS
+
+def write_fits(path: str, m: np.ndarray, nside: int) -> None:
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    _check(load_library().dmann_write_healpix_fits(path.encode(), C.c_void_p(m.ctypes.data), m.size, nside))
+
+
+def read_fits(path: str):
+    L = load_library()
+    ns, n = C.c_int(), C.c_int64()
+    _check(L.dmann_read_healpix_fits(path.encode(), None, 0, C.byref(ns), C.byref(n)))
+    out = np.empty(n.value, dtype=np.float64)
+    _check(L.dmann_read_healpix_fits(path.encode(), C.c_void_p(out.ctypes.data), n.value, C.byref(ns),
+                                     C.byref(n)))
+    return out, ns.value

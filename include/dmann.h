@@ -1,0 +1,169 @@
+/*
+ * dmann.h -- C ABI of the MI355X annihilation-map projector (libdmann.so).
+ *
+ * Drop-in for the device API of the reference CUDA render path
+ * (src/chealpixrender.h:60-64, implemented at src/chealpixrender.cu:33-125):
+ *
+ *   reference (C++ linkage, module-global state)      this library
+ *   ---------------------------------------------      -----------------------------
+ *   initializeCUDA(nside, numofparts)   cu:33-80   ->  dmann_create
+ *   calculateMapByGPU(renderpart*, n)   cu:83-103  ->  dmann_render / dmann_render_device
+ *   getCUDAMap(MAPTYPE* map)            cu:105-120 ->  dmann_get_map
+ *   cudaCleaingUp()                     cu:122-125 ->  dmann_destroy
+ *
+ * Differences by design:
+ *  - The caller passes raw 48-byte DMParticle records (src/datatypes.h:46-105),
+ *    not host-preprocessed `renderpart`s: the per-particle pre-pass of
+ *    src/genmap_cuda_render.cpp:102-167 (filter, distance, flux, angles,
+ *    angular radius, sub-pixel rule) runs on the device.
+ *  - The map follows the CPU generator's semantics (src/genmap.cpp:22-175):
+ *    footprint = pixel centres within 2h/d (HEALPix query_disc), single-pixel
+ *    deposit when 2h/d < theta0 or the footprint has < 2 pixels, Gaussian
+ *    weight exp(-0.5 (psi/r)^2 / 0.333) normalised over the footprint.
+ *  - State lives in an opaque context (re-entrant; one context per map).
+ *  - Every entry point returns a status (0 = success) and never exits the
+ *    process; dmann_last_error() gives the message.
+ *  - Renders accumulate into the context's map across calls, as successive
+ *    calculateMapByGPU calls accumulate into d_map.
+ */
+#ifndef DMANN_H
+#define DMANN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 48-byte particle record; field order of DMParticle (src/datatypes.h:48-60). */
+typedef struct dmann_particle {
+    float mass, dens, hsmooth;
+    float posx, posy, posz;
+    float velx, vely, velz;
+    float sigmav, eps, phi;
+} dmann_particle;
+
+/* Accumulation / arithmetic form of a context. */
+enum dmann_precision {
+    DMANN_PREC_FP32 = 0,        /* FP32 weights, plain FP32 atomic accumulation          */
+    DMANN_PREC_COMPENSATED = 1, /* FP32 weights, two-float (hi+lo) compensated accumulation */
+    DMANN_PREC_FP64 = 2         /* FP64 weights as the reference computes them; FP64 map  */
+};
+
+/* Status codes. */
+enum dmann_status {
+    DMANN_OK = 0,
+    DMANN_ERR_ARG = 1,      /* bad argument (null pointer, nside, sizes)          */
+    DMANN_ERR_CAPACITY = 2, /* more particles than the context was created for   */
+    DMANN_ERR_HIP = 3,      /* a HIP runtime call failed                          */
+    DMANN_ERR_DOMAIN = 4    /* a kept particle has an undefined direction (d == 0
+                               or |cos theta| > 1); the reference exits here
+                               (src/anglefuncs.cpp:60-63)                          */
+};
+
+/* Per-map constants, computed on the host from Parameters.
+ *  opos       params.opos (src/parameters.cpp:121)
+ *  rotmatrix  rotmatrix[i][j] row-major (src/parameters.cpp:183-289)
+ *  flux_factor  unit_factor of src/flux/flux_ann_nosom.cpp:13-16
+ *  mass_cut   (double)(float)min positive particle mass * 1.1; particles with
+ *             mass >= mass_cut or dens < 0 are skipped (src/genmap.cpp:52-58,82-85).
+ *             Use FLT_MAX*1.1 when no masses are configured.
+ *  nside      HEALPix nside, 1..8192 (src/chealpixrender.cu:25 NS_MAX)
+ *  precision  enum dmann_precision                                           */
+typedef struct dmann_params {
+    double opos[3];
+    double rotmatrix[9];
+    double flux_factor;
+    double mass_cut;
+    int32_t nside;
+    int32_t precision;
+} dmann_params;
+
+/* Counters of the last dmann_get_stats call (cumulative since create/reset). */
+typedef struct dmann_stats {
+    int64_t kept;          /* particles passing the filter                        */
+    int64_t subpixel;      /* deposited whole because 2r < theta0                 */
+    int64_t fallback;      /* deposited whole because the footprint had < 2 pixels */
+    int64_t extended;      /* spread over their footprint                         */
+    int64_t contributions; /* total footprint pixels of extended particles        */
+    int64_t large;         /* extended particles rendered by the sky-covering path */
+} dmann_stats;
+
+typedef struct dmann_ctx dmann_ctx;
+
+/* Reads a reference ini file (key = value; src/parameters.cpp:310-470) and
+ * fills *p with its map constants: observer, rotation matrix
+ * (src/parameters.cpp:111-289), the nosom flux factor
+ * (src/flux/flux_ann_nosom.cpp:13-16), the low-resolution mass cut
+ * (src/genmap.cpp:52-58) and nside; `precision` is copied.  mem_buffer and
+ * test_num (may be NULL) receive the ini's mem_buffer and test values.
+ * Returns DMANN_ERR_ARG when the file cannot be read (where the reference's
+ * Parameters constructor exits, src/parameters.cpp:471-474). */
+int dmann_params_from_ini(const char *ini, int precision, dmann_params *p, int *mem_buffer, int *test_num);
+
+/* Creates a context on HIP device `device` for maps of p->nside, accepting up
+ * to max_parts particles per render call.  The map starts at zero.
+ * Replaces initializeCUDA (src/chealpixrender.cu:33-80). */
+int dmann_create(dmann_ctx **ctx, const dmann_params *p, int64_t max_parts, int device);
+
+/* Renders n host-resident particles (copied to the device first) and adds
+ * them to the map.  Replaces calculateMapByGPU (src/chealpixrender.cu:83-103). */
+int dmann_render(dmann_ctx *ctx, const dmann_particle *host_parts, int64_t n);
+
+/* Same for particles already resident in device memory.  Asynchronous on the
+ * context's stream; the pointer must stay valid until the next synchronising
+ * call (dmann_get_map, dmann_synchronize). */
+int dmann_render_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t n);
+
+/* Uses `stream` (a hipStream_t, NULL = the null stream) for later calls. */
+int dmann_set_stream(dmann_ctx *ctx, void *stream);
+
+/* Waits for the context's queued work.  Reports DMANN_ERR_DOMAIN when a kept
+ * particle had an undefined direction. */
+int dmann_synchronize(dmann_ctx *ctx);
+
+/* Writes the finished map, RING order, npix doubles, already divided by
+ * dOmega = 4 pi / npix (src/genmap.cpp:164-169), to host memory.
+ * Replaces getCUDAMap (src/chealpixrender.cu:105-120) plus the host combine
+ * of src/genmap_cuda_render.cpp:191-196. */
+int dmann_get_map(dmann_ctx *ctx, double *host_map);
+
+/* Same into device memory (npix doubles); asynchronous on the stream. */
+int dmann_get_map_device(dmann_ctx *ctx, double *dev_map);
+
+/* Sets the map and the counters back to zero. */
+int dmann_reset(dmann_ctx *ctx);
+
+int dmann_get_stats(dmann_ctx *ctx, dmann_stats *st);
+
+/* Number of pixels of the context's map (12 nside^2). */
+int64_t dmann_npix(const dmann_ctx *ctx);
+
+/* Frees everything.  Replaces cudaCleaingUp (src/chealpixrender.cu:122-125). */
+void dmann_destroy(dmann_ctx *ctx);
+
+/* HEALPix FITS map file, as MapGenerator::saveToFits writes it
+ * (src/mapgenerator.cpp:30-62): RING order, one float64 column.
+ * Writes npix doubles of `map` for `nside`. */
+int dmann_write_healpix_fits(const char *path, const double *map, int64_t npix, int nside);
+/* Reads such a file; with map == NULL only *nside / *npix are filled. */
+int dmann_read_healpix_fits(const char *path, double *map, int64_t cap, int *nside, int64_t *npix);
+
+/* Host-side footprint of one particle, computed with the same code the
+ * kernels run (no GPU involved): pixels (RING) and unnormalised FP32 weights
+ * of the particle's footprint in the kernels' ring order, up to cap entries.
+ * *kind: 0 filtered, 1 sub-pixel (whole flux to pix[0]), 2 footprint of < 2
+ * pixels (whole flux to pix[0]), 3 spread.  Returns the footprint size (1
+ * for kinds 1 and 2), 0 when filtered, < 0 on bad input or an undefined
+ * direction.  A debugging and test aid with no counterpart in the reference. */
+int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int32_t *pix, float *w,
+                             int64_t cap, double *flux, int *kind);
+
+/* Message of the last failed call on ctx (or of the last failed
+ * dmann_create when ctx is NULL). */
+const char *dmann_last_error(const dmann_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMANN_H */

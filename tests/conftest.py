@@ -1,0 +1,60 @@
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "dm-annihilation-cxx_amd")
+ORACLE = os.path.join(ROOT, "oracle")
+for p in (PKG, ORACLE):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a ROCm GPU (MI355X)")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import oracle
+    if not os.path.exists(oracle.LIB_PATH):
+        subprocess.run(["make", "-C", ORACLE, "-s", "liboracle.so"], check=True)
+    oracle.lib()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def pw():
+    import pywrapper
+    pywrapper.load_library()
+    return pywrapper
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def rel_err_report(gpu_map, ref_map, floor=1e-6):
+    """Max per-pixel relative error over pixels above floor*max, and the
+    relative error of the total."""
+    ref = np.asarray(ref_map, dtype=np.float64)
+    got = np.asarray(gpu_map, dtype=np.float64)
+    mx = np.abs(ref).max() if ref.size else 0.0
+    sel = np.abs(ref) > floor * mx
+    if sel.any():
+        per = np.abs(got[sel] - ref[sel]) / np.abs(ref[sel])
+        max_rel = float(per.max())
+    else:
+        max_rel = 0.0
+    # pixels below the floor must still be small in absolute terms
+    below = float(np.abs(got[~sel] - ref[~sel]).max() / mx) if (~sel).any() and mx > 0 else 0.0
+    tot_ref = ref.sum()
+    tot_rel = float(abs(got.sum() - tot_ref) / abs(tot_ref)) if tot_ref != 0 else float(abs(got.sum()))
+    return {"max_rel": max_rel, "total_rel": tot_rel, "below_floor_abs": below, "n_sel": int(sel.sum())}
