@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "dmann.h"
 #include "dmann_footprint.h"
@@ -32,8 +33,16 @@ struct dmann_ctx {
     MapRef map{nullptr, nullptr, nullptr};
     double *d_out = nullptr; // finalised FP64 map for host reads
     LaunchCfg cfg{};
+    bool timing = false;
+    std::vector<hipEvent_t> events; // 4 per timed render call
     char msg[512] = {0};
 };
+
+static void drop_events(dmann_ctx *c)
+{
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    c->events.clear();
+}
 
 static char g_create_msg[512] = "";
 
@@ -159,8 +168,39 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
         return fail(c, DMANN_ERR_CAPACITY, "%lld particles exceed the context capacity %lld", (long long)n,
                     (long long)c->max_parts);
     HIPCHK(c, hipSetDevice(c->device));
+    hipEvent_t *ev = nullptr;
+    if (c->timing) {
+        for (int i = 0; i < 4; i++) {
+            hipEvent_t e;
+            HIPCHK(c, hipEventCreate(&e));
+            c->events.push_back(e);
+        }
+        ev = &c->events[c->events.size() - 4];
+    }
     HIPCHK(c, launch_render(c->acc, reinterpret_cast<const Particle *>(dev_parts), n, c->P, c->d_recs,
-                            c->d_counts, c->map, c->d_stats, c->cfg, c->stream));
+                            c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev));
+    return DMANN_OK;
+}
+
+int dmann_timing(dmann_ctx *c, int enable, double *ms, int *calls)
+{
+    if (!c) return DMANN_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (ms || calls) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double acc[3] = {0, 0, 0};
+        for (size_t i = 0; i + 3 < c->events.size(); i += 4)
+            for (int k = 0; k < 3; k++) {
+                float t = 0;
+                HIPCHK(c, hipEventElapsedTime(&t, c->events[i + k], c->events[i + k + 1]));
+                acc[k] += t;
+            }
+        if (ms)
+            for (int k = 0; k < 3; k++) ms[k] = acc[k];
+        if (calls) *calls = (int)(c->events.size() / 4);
+    }
+    drop_events(c);
+    c->timing = enable != 0;
     return DMANN_OK;
 }
 
@@ -247,6 +287,7 @@ void dmann_destroy(dmann_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    drop_events(c);
     free_all(c);
     delete c;
 }

@@ -108,6 +108,8 @@ DMANN_HD PGeo particle_geo(const DiscRec &rec, int nside)
 struct RingRec {
     int cnt;
     float A, B, offm, step; // haversine terms (FP32)
+    float Ac;               // cos^2((theta_ring - theta)/2)
+    float E, D;             // sin^2, cos^2 of (theta_ring + theta)/2
     int kc_rel;             // column nearest phi, relative to col0
     int col0, startpix, nr;
     double z, s, shift;     // FP64 terms (FP64 weight form)
@@ -158,11 +160,21 @@ DMANN_HD RingRec ring_setup(const PGeo &g, int nside, int gi)
         }
     }
     const double s = sqrt((1 - z) * (1 + z));
-    double A = (1.0 - (z * g.z0 + s * g.s0)) * 0.5; // hav(theta_ring - theta)
+    const double cdt = z * g.z0 + s * g.s0;         // cos(theta_ring - theta)
+    double A = (1.0 - cdt) * 0.5;                   // hav(theta_ring - theta)
+    double Ac = (1.0 + cdt) * 0.5;                  // 1 - hav(theta_ring - theta)
     if (A < 0) A = 0;
+    if (Ac < 0) Ac = 0;
+    const double cst = z * g.z0 - s * g.s0;         // cos(theta_ring + theta)
+    double E = (1.0 - cst) * 0.5, D = (1.0 + cst) * 0.5;
+    if (E < 0) E = 0;
+    if (D < 0) D = 0;
     const double step = kTwoPi / fnr;
     o.cnt = cnt;
     o.A = (float)A;
+    o.Ac = (float)Ac;
+    o.E = (float)E;
+    o.D = (float)D;
     o.B = (float)(s * g.s0);
     o.offm = (float)(((double)kc + shift) * step - g.phi);
     o.step = (float)step;
@@ -201,12 +213,20 @@ DMANN_HD float fast_exp2(float x)
 #endif
 }
 
-// Gaussian weight of a pixel at longitude offset x from the particle in a
-// ring with haversine terms (A, B): h = hav(psi) = A + B sin^2(x/2),
-// psi/2 = asin(sqrt h), weight = exp(-0.5 (psi/r)^2 / 0.333)
-// (src/genmap.cpp:137-143) = exp2(K4 asin^2(sqrt h)).
+// Gaussian weight of a pixel at longitude offset x from the particle, in a
+// ring at colatitude theta_r; theta is the particle's colatitude:
+//   h = hav(psi) = A + B sin^2(x/2),  A = sin^2((theta_r-theta)/2),
+//   B = sin(theta_r) sin(theta);  weight = exp(-0.5 (psi/r)^2 / 0.333)
+//   (src/genmap.cpp:137-143) = exp2(K4 (psi/2)^2).
+// MODE 0 (psi <= 0.5, |x| <= 0.5): (psi/2)^2 = h g(h) by polynomials.
+// MODE 1 (any geometry): with hp = sin^2(x/2), cp = cos^2(x/2),
+//   h = A cp + E hp and 1 - h = Ac cp + D hp (E, D = sin^2, cos^2 of
+//   (theta_r+theta)/2; Ac = 1 - A): sums of non-negative terms, so
+//   psi/2 = atan2(sqrt h, sqrt(1-h)) keeps its relative accuracy up to the
+//   antipode.  The general form takes (A, Ac, E, D) in (A, B, Ac, D) with
+//   E passed in B.
 template <int MODE>
-DMANN_HD float weight32(float x, float A, float B, float K4)
+DMANN_HD float weight32(float x, float A, float B, float Ac, float D, float K4)
 {
     if (MODE == 0) {
         // sin^2(x/2) = u (1/4 - u/48 + u^2/1440 - u^3/80640), u = x^2, |x| <= 0.5
@@ -224,10 +244,12 @@ DMANN_HD float weight32(float x, float A, float B, float K4)
         gq = fmaf(h, gq, 1.0f);
         return fast_exp2(K4 * (h * gq));
     } else {
-        const float sh = sinf(0.5f * x);
-        float h = fmaf(B, sh * sh, A);
-        h = fminf(h, 1.0f);
-        const float a = asinf(sqrtf(h));
+        float sh, ch;
+        sincosf(0.5f * x, &sh, &ch);
+        const float hp = sh * sh, cp = ch * ch;
+        const float h = fmaf(B, hp, A * cp);
+        const float c = fmaf(D, hp, Ac * cp);
+        const float a = atan2f(sqrtf(h), sqrtf(c));
         return fast_exp2(K4 * (a * a));
     }
 }
@@ -236,7 +258,8 @@ DMANN_HD float ring_weight32(const PGeo &g, const RingRec &rr, int t)
 {
     const float kf = (float)(t - rr.kc_rel);
     const float x = fmaf(kf, rr.step, rr.offm);
-    return g.mode == 0 ? weight32<0>(x, rr.A, rr.B, g.K4) : weight32<1>(x, rr.A, rr.B, g.K4);
+    return g.mode == 0 ? weight32<0>(x, rr.A, rr.B, 0.0f, 0.0f, g.K4)
+                       : weight32<1>(x, rr.A, rr.E, rr.Ac, rr.D, g.K4);
 }
 
 } // namespace dmann

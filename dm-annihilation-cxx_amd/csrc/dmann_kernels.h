@@ -65,9 +65,11 @@ struct LaunchCfg {
     int large_blocks;
 };
 
+// ev (optional): 4 events recorded before the pre-pass, after it, after the
+// sky-covering kernel and after the medium kernel.
 hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                          unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
-                         hipStream_t st);
+                         hipStream_t st, hipEvent_t *ev = nullptr);
 hipError_t launch_finalize(int acc, MapRef map, int64_t npix, double dOmega, double *out, hipStream_t st);
 hipError_t launch_clear(int acc, MapRef map, int64_t npix, unsigned long long *stats, hipStream_t st);
 

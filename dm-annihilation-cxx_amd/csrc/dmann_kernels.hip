@@ -166,6 +166,7 @@ struct WinLDS {
     int pre_end[64];
     int kref[64];     // pre_start + kc_rel
     float4 f[64];     // A, B, offm, step
+    float4 gq[64];    // E, Ac, D (general weight form)
     int4 a[64];       // abase, k_wrap, nr, kcol (= pre_start - col0)
     double4 d[64];    // z, s, shift, 2pi/nr  (FP64 form only)
 };
@@ -190,6 +191,7 @@ __device__ __forceinline__ int setup_window(const PGeo &g, int nside, int win, W
     if (gi < g.nrings) {
         L.kref[lane] = pre_start + rr.kc_rel;
         L.f[lane] = make_float4(rr.A, rr.B, rr.offm, rr.step);
+        L.gq[lane] = make_float4(rr.E, rr.Ac, rr.D, 0.0f);
         int abase, kw;
         ring_addr(rr, pre_start, abase, kw);
         L.a[lane] = make_int4(abase, kw, rr.nr, pre_start - rr.col0);
@@ -200,6 +202,18 @@ __device__ __forceinline__ int setup_window(const PGeo &g, int nside, int win, W
 }
 
 __device__ __forceinline__ int pix_addr(const int4 a, int k) { return ring_pixel(a.x, a.y, a.z, k); }
+
+// FP32 weight of flat pixel k in ring j of the window (see weight32).
+template <int MODE>
+__device__ __forceinline__ float wt32(float x, const float4 &f, const WinLDS &L, int j, float K4)
+{
+    if constexpr (MODE == 0) {
+        return weight32<0>(x, f.x, f.y, 0.0f, 0.0f, K4);
+    } else {
+        const float4 q = L.gq[j];
+        return weight32<1>(x, f.x, q.x, q.y, q.z, K4);
+    }
+}
 
 // Register capacity of the single-evaluation path: footprints of up to
 // 64*kNW pixels keep their weights in VGPRs between the norm and the deposit.
@@ -232,7 +246,7 @@ __device__ int render_particle_wave(const PGeo &g, int nside, WinLDS &L, int lan
                         while (L.pre_end[j] <= k) ++j;
                         const float4 f = L.f[j];
                         const float kf = (float)(k - L.kref[j]);
-                        const float w = weight32<MODE>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                        const float w = wt32<MODE>(fmaf(kf, f.w, f.z), f, L, j, g.K4);
                         wv[it] = w;
                         av[it] = pix_addr(L.a[j], k);
                         nsum += w;
@@ -272,7 +286,7 @@ __device__ int render_particle_wave(const PGeo &g, int nside, WinLDS &L, int lan
                 while (L.pre_end[j] <= k) ++j;
                 const float4 f = L.f[j];
                 const float kf = (float)(k - L.kref[j]);
-                part += weight32<MODE>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                part += wt32<MODE>(fmaf(kf, f.w, f.z), f, L, j, g.K4);
                 if (++c == 16) {
                     nacc += (double)part;
                     part = 0.0f;
@@ -306,7 +320,7 @@ __device__ int render_particle_wave(const PGeo &g, int nside, WinLDS &L, int lan
             } else {
                 const float4 f = L.f[j];
                 const float kf = (float)(k - L.kref[j]);
-                const float w = weight32<MODE>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                const float w = wt32<MODE>(fmaf(kf, f.w, f.z), f, L, j, g.K4);
                 acc_add<ACC>(map, addr, w * scale, 0.0);
             }
         }
@@ -377,6 +391,7 @@ __global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ 
                     const int cnt = L.pre_end[j] - pre_start;
                     if (cnt <= 0) continue;
                     const float4 f = L.f[j];
+                    const float4 gq = L.gq[j];
                     const int kref = L.kref[j];
                     const int4 a = L.a[j];
                     const double4 dd = L.d[j];
@@ -390,8 +405,8 @@ __global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ 
                         } else {
                             const float kf = (float)(k - kref);
                             float w;
-                            if (g.mode == 0) w = weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
-                            else w = weight32<1>(fmaf(kf, f.w, f.z), f.x, f.y, g.K4);
+                            if (g.mode == 0) w = weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, 0.0f, 0.0f, g.K4);
+                            else w = weight32<1>(fmaf(kf, f.w, f.z), f.x, gq.x, gq.y, gq.z, g.K4);
                             if (pass == 0) part += w;
                             else acc_add<ACC>(map, pix_addr(a, k), w * (float)scale64, 0.0);
                         }
@@ -473,26 +488,30 @@ __global__ void k_finalize(MapRef map, int64_t npix, double dOmega, double *__re
 template <int ACC>
 static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                                   unsigned *counts, MapRef map, unsigned long long *stats,
-                                  const LaunchCfg &cfg, hipStream_t st)
+                                  const LaunchCfg &cfg, hipStream_t st, hipEvent_t *ev)
 {
     k_fill_u32<<<1, 64, 0, st>>>(counts, 2);
     const int64_t want = (n + 255) / 256;
     const int pre_blocks = (int)(want < cfg.prepass_blocks ? want : cfg.prepass_blocks);
+    if (ev) (void)hipEventRecord(ev[0], st);
     if (pre_blocks > 0)
         k_prepass<ACC><<<pre_blocks, 256, 0, st>>>(d_parts, n, P, recs, counts, map, stats);
+    if (ev) (void)hipEventRecord(ev[1], st);
     k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, map, stats);
+    if (ev) (void)hipEventRecord(ev[2], st);
     k_disc_medium<ACC><<<cfg.medium_blocks, 256, 0, st>>>(recs, counts, P.nside, map, stats);
+    if (ev) (void)hipEventRecord(ev[3], st);
     return hipGetLastError();
 }
 
 hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                          unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
-                         hipStream_t st)
+                         hipStream_t st, hipEvent_t *ev)
 {
     switch (acc) {
-    case kAccF32: return launch_render_t<kAccF32>(d_parts, n, P, recs, counts, map, stats, cfg, st);
-    case kAccComp: return launch_render_t<kAccComp>(d_parts, n, P, recs, counts, map, stats, cfg, st);
-    default: return launch_render_t<kAccF64>(d_parts, n, P, recs, counts, map, stats, cfg, st);
+    case kAccF32: return launch_render_t<kAccF32>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev);
+    case kAccComp: return launch_render_t<kAccComp>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev);
+    default: return launch_render_t<kAccF64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev);
     }
 }
 

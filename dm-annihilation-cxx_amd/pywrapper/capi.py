@@ -50,7 +50,7 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_set_stream", "dmann_synchronize", "dmann_get_map", "dmann_get_map_device",
            "dmann_reset", "dmann_get_stats", "dmann_npix", "dmann_destroy",
            "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint",
-           "dmann_last_error")
+           "dmann_timing", "dmann_last_error")
 
 _lib = None
 
@@ -90,6 +90,7 @@ def load_library():
     L.dmann_host_footprint.argtypes = [C.POINTER(DmannParams), vp, vp, vp, i64, C.POINTER(C.c_double),
                                        C.POINTER(C.c_int)]
     L.dmann_host_footprint.restype = i64
+    L.dmann_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.dmann_last_error.argtypes = [vp]
     L.dmann_last_error.restype = C.c_char_p
     _lib = L
@@ -211,6 +212,15 @@ class Projector:
         self._bind_stream()
         _check(self._L.dmann_get_map_device(self._h, C.c_void_p(out.data_ptr())), self._h)
         return out
+
+    def timing(self, enable: bool):
+        """Returns ({'prepass','large','medium'}: summed ms, calls) of the
+        render calls since the last timing() call, then enables/disables
+        event recording for the next ones."""
+        ms = (C.c_double * 3)()
+        calls = C.c_int()
+        _check(self._L.dmann_timing(self._h, int(bool(enable)), ms, C.byref(calls)), self._h)
+        return {"prepass": ms[0], "large": ms[1], "medium": ms[2]}, calls.value
 
     def reset(self) -> None:
         _check(self._L.dmann_reset(self._h), self._h)

@@ -122,6 +122,14 @@ int dmann_set_stream(dmann_ctx *ctx, void *stream);
  * particle had an undefined direction. */
 int dmann_synchronize(dmann_ctx *ctx);
 
+/* Per-kernel timing with HIP events on the context's stream.  Reads the
+ * summed durations (ms) of the kernels of the render calls made since the
+ * last call -- ms[0] pre-pass, ms[1] sky-covering kernel, ms[2] medium
+ * kernel -- and their number (both may be NULL), then turns event recording
+ * on (enable != 0) or off for the following render calls.  A measurement
+ * aid with no counterpart in the reference. */
+int dmann_timing(dmann_ctx *ctx, int enable, double *ms, int *calls);
+
 /* Writes the finished map, RING order, npix doubles, already divided by
  * dOmega = 4 pi / npix (src/genmap.cpp:164-169), to host memory.
  * Replaces getCUDAMap (src/chealpixrender.cu:105-120) plus the host combine

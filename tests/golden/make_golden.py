@@ -1,0 +1,110 @@
+#!/usr/bin/env python3
+"""Writes the golden known-answer fixtures of tests/golden/ from the
+reference's own compiled code (oracle/_ref/libref_kat.so, built by
+`make -C oracle ref` from src/parameters.cpp, src/datatypes.cpp,
+src/flux/flux_ann_nosom.cpp, src/datareader.cpp, src/tipsy_io.c).
+
+Fixtures:
+  params_kat.json   Parameters outputs (rotation matrix, observer, unit
+                    chain, masses, map) and getflux values, full precision,
+                    for the ini files below.
+  reader_kat.npz    a small native particle file and mask, and the records
+                    DataReader returns for several (test, mem_buffer) settings.
+  VL2_info.ini, test_VL2_info.ini
+                    the reference's own ini fixtures (src/VL2_info.ini,
+                    src/test/VL2_info.ini), inputs of the KAT.
+  synth_obs8.ini, synth_offaxis.ini
+                    ini files of the synthetic workloads (written here).
+Run in a container that has the reference tree; the fixtures are committed.
+"""
+import json
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "dm-annihilation-cxx_amd"))
+
+import oracle  # noqa: E402
+
+REF = "/root/reference/src"
+NAMES = (["rot%d" % i for i in range(9)] + ["opos%d" % i for i in range(3)] + ["cpos%d" % i for i in range(3)]
+         + ["otheta", "ophi", "mass_to_cgs", "mass_to_Msun", "length_to_Mpc", "length_to_cgs", "time_to_cgs",
+            "velocity_to_cgs", "density_to_cgs", "annihilation_flux_to_cgs", "rho_crit_in_cgs", "Delta_vir"]
+         + ["particle_masses%d" % i for i in range(10)]
+         + ["Nside", "Npix", "dOmega", "memParts", "testNum", "isNative", "isMask"])
+
+
+def write_synth_inis():
+    from pywrapper.capi import write_ini
+    write_ini(os.path.join(HERE, "synth_obs8.ini"), nside=64)
+    write_ini(os.path.join(HERE, "synth_offaxis.ini"), nside=256, obs_pos=(3.0, -4.5, 6.25),
+              centerpos=(0.5, 0.25, -0.75), align_vec=(0.3, -0.2, 0.93), particle_masses=(4100.0, 8200.0, 0.0),
+              mem_buffer=4096)
+
+
+def main():
+    oracle.build()
+    R = oracle.ref()
+    if R is None:
+        sys.exit("oracle/_ref/libref_kat.so is not built (needs the reference tree)")
+    shutil.copyfile(os.path.join(REF, "VL2_info.ini"), os.path.join(HERE, "VL2_info.ini"))
+    shutil.copyfile(os.path.join(REF, "test", "VL2_info.ini"), os.path.join(HERE, "test_VL2_info.ini"))
+    write_synth_inis()
+    inis = ["VL2_info.ini", "test_VL2_info.ini", "synth_obs8.ini", "synth_offaxis.ini"]
+    rng = np.random.default_rng(12345)
+    recs = np.zeros(8, dtype=oracle.PARTICLE_DTYPE)
+    recs["mass"] = rng.uniform(1e-9, 5e3, 8)
+    recs["dens"] = rng.uniform(1e-3, 1e6, 8)
+    recs["hsmooth"] = rng.uniform(0.01, 10, 8)
+    dists = rng.uniform(0.05, 300.0, 8)
+    out = {"fields": NAMES, "cases": {}}
+    for ini in inis:
+        path = os.path.join(HERE, ini)
+        v = np.zeros(44)
+        R.ref_params(path.encode(), v)
+        flux = []
+        for r, d in zip(recs, dists):
+            f32 = np.ascontiguousarray(np.array([r], dtype=oracle.PARTICLE_DTYPE)).view(np.float32).copy()
+            flux.append(R.ref_getflux(path.encode(), f32, float(d)))
+        out["cases"][ini] = {"values": [float(x).hex() for x in v],
+                             "flux_records": recs.view(np.float32).reshape(8, 12).tolist(),
+                             "flux_dists": [float(d).hex() for d in dists],
+                             "flux": [float(f).hex() for f in flux]}
+    with open(os.path.join(HERE, "params_kat.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+    # native reader KAT
+    n = 1000
+    parts = np.zeros(n, dtype=oracle.PARTICLE_DTYPE)
+    for i, f in enumerate(oracle.PARTICLE_DTYPE.names):
+        parts[f] = rng.standard_normal(n).astype(np.float32) * (i + 1)
+    mask = (rng.random(n) > 0.3).astype(np.uint8)
+    tmp = os.path.join(HERE, "_tmp_native.bin")
+    tmpm = os.path.join(HERE, "_tmp_mask.bin")
+    with open(tmp, "wb") as fh:
+        np.array([n], dtype="<i4").tofile(fh)
+        parts.tofile(fh)
+    with open(tmpm, "wb") as fh:
+        np.array([n], dtype="<i4").tofile(fh)
+        mask.tofile(fh)
+    cases = {}
+    for name, test, membuf, use_mask in (("all_chunk97", -1, 97, False), ("test333_chunk50", 333, 50, False),
+                                         ("mask_chunk128", -1, 128, True), ("mask_test700_chunk1000", 700, 1000, True)):
+        buf = np.zeros(n * 12, dtype=np.float32)
+        got = R.ref_read_native(tmp.encode(), test, membuf, (tmpm if use_mask else "").encode(), buf, n)
+        cases[name] = (test, membuf, use_mask, buf[:got * 12].reshape(got, 12).copy())
+    np.savez_compressed(os.path.join(HERE, "reader_kat.npz"), particles=parts.view(np.float32).reshape(n, 12),
+                        mask=mask, **{f"{k}__out": v[3] for k, v in cases.items()},
+                        **{f"{k}__cfg": np.array([v[0], v[1], int(v[2])]) for k, v in cases.items()})
+    os.unlink(tmp)
+    os.unlink(tmpm)
+    print("wrote", os.listdir(HERE))
+
+
+if __name__ == "__main__":
+    main()

@@ -4,7 +4,10 @@ Tolerances (per-pixel relative error over pixels above 1e-6 of the map
 maximum; total-flux relative error):
   fp64          1e-9 / 1e-12   (reference arithmetic on the device)
   compensated   1e-5 / 1e-6    (FP32 weights, two-float accumulation)
-  fp32          1e-5 / 1e-6    (FP32 weights, FP32 atomics; at these sizes)
+  fp32          2e-5 / 1e-5    (FP32 weights, plain FP32 atomics: it loses
+                                precision with the number of contributions
+                                per pixel, which is what the compensated
+                                form wins back; its error is reported)
 Integer work (which particles are kept / sub-pixel / fallback / extended and
 the exact footprint pixel counts) must match the oracle exactly.
 """
@@ -15,7 +18,7 @@ from conftest import rel_err_report
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp64": (1e-9, 1e-12), "compensated": (1e-5, 1e-6), "fp32": (1e-5, 1e-6)}
+TOL = {"fp64": (1e-9, 1e-12), "compensated": (1e-5, 1e-6), "fp32": (2e-5, 1e-5)}
 
 
 def _run_gpu(pw, recs, nside, precision, obs=(8.0, 0.0, 0.0), chunks=1, **ini):

@@ -1,0 +1,113 @@
+"""Pins the oracle's and the product's Parameters / getflux restatements.
+
+Golden vectors: tests/golden/params_kat.json, produced by the reference's own
+src/parameters.cpp + src/flux/flux_ann_nosom.cpp compiled here
+(tests/golden/make_golden.py).  Bit-exact equality is required: the unit
+chain and rotation are pure FP64 arithmetic in a fixed order.
+Closed forms pin the observer=(8,0,0), centre=0, align=(0,0,1) geometry.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _kat():
+    with open(os.path.join(GOLD, "params_kat.json")) as fh:
+        return json.load(fh)
+
+
+def _oracle_values(p):
+    return (list(p.rotmatrix) + list(p.opos) + list(p.cpos) + [p.otheta, p.ophi, p.mass_to_cgs, p.mass_to_Msun,
+            p.length_to_Mpc, p.length_to_cgs, p.time_to_cgs, p.velocity_to_cgs, p.density_to_cgs,
+            p.annihilation_flux_to_cgs, p.rho_crit_in_cgs, p.Delta_vir] + list(p.particle_masses)
+            + [p.nside, p.npix, p.dOmega, p.mem_buffer, p.test, p.is_native, p.is_mask])
+
+
+@pytest.mark.parametrize("ini", ["VL2_info.ini", "test_VL2_info.ini", "synth_obs8.ini", "synth_offaxis.ini"])
+def test_oracle_params_match_reference_kat(orc, ini):
+    kat = _kat()
+    want = [float.fromhex(x) for x in kat["cases"][ini]["values"]]
+    got = _oracle_values(orc.params_from_ini(os.path.join(GOLD, ini)))
+    bad = [(n, a, b) for n, a, b in zip(kat["fields"], want, got) if a != b]
+    assert not bad
+
+
+@pytest.mark.parametrize("ini", ["VL2_info.ini", "test_VL2_info.ini", "synth_obs8.ini", "synth_offaxis.ini"])
+def test_oracle_getflux_matches_reference_kat(orc, ini):
+    kat = _kat()["cases"][ini]
+    p = orc.params_from_ini(os.path.join(GOLD, ini))
+    recs = np.array(kat["flux_records"], dtype=np.float32).view(orc.PARTICLE_DTYPE).reshape(-1)
+    for rec, d, f in zip(recs, kat["flux_dists"], kat["flux"]):
+        assert orc.getflux(p.flux_factor, rec, float.fromhex(d)) == float.fromhex(f)
+
+
+@pytest.mark.parametrize("ini", ["VL2_info.ini", "test_VL2_info.ini", "synth_obs8.ini", "synth_offaxis.ini"])
+def test_product_params_match_reference_kat(pw, ini):
+    """The library's own Parameters (C ABI dmann_params_from_ini)."""
+    kat = _kat()
+    want = dict(zip(kat["fields"], [float.fromhex(x) for x in kat["cases"][ini]["values"]]))
+    p, mem, test = pw.params_from_ini(os.path.join(GOLD, ini), "fp64")
+    assert list(p.rotmatrix) == [want["rot%d" % i] for i in range(9)]
+    assert list(p.opos) == [want["opos%d" % i] for i in range(3)]
+    assert p.nside == want["Nside"]
+    assert mem == want["memParts"] and test == want["testNum"]
+    # flux factor: the reference's getflux at d = 1/sqrt(4 pi) with dens = mass = 1
+    rec = np.zeros(12, dtype=np.float32)
+    rec[0] = rec[1] = 1.0
+    kat_case = kat["cases"][ini]
+    recs = np.array(kat_case["flux_records"], dtype=np.float32)
+    for r, d, f in zip(recs, kat_case["flux_dists"], kat_case["flux"]):
+        d = float.fromhex(d)
+        mine = p.flux_factor * float(r[1]) * float(r[0]) / (4.0 * math.pi * d * d)
+        assert mine == float.fromhex(f)
+    # mass cut: (double)(float)min positive mass * 1.1
+    masses = [want["particle_masses%d" % i] for i in range(10)]
+    pos = [m for m in masses if m > 0]
+    hires = np.float32(min(pos)) if pos else np.float32(np.finfo(np.float32).max)
+    assert p.mass_cut == float(hires) * 1.1
+
+
+def test_rotation_closed_form_observer_on_x_axis(orc):
+    """Observer (8,0,0), centre 0, align (0,0,1): the halo centre maps to
+    theta = pi/2, phi = pi; +z to (pi/2, pi/2); +y to theta = pi
+    (src/parameters.cpp:183-289 with src/anglefuncs.cpp:15-52)."""
+    p = orc.params_from_ini(os.path.join(GOLD, "synth_obs8.ini"))
+    R = np.array(list(p.rotmatrix)).reshape(3, 3)
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-15)
+    assert np.isclose(np.linalg.det(R), 1.0, atol=1e-15)
+
+    def ang(v):
+        v = np.asarray(v, float)
+        ct, ph = orc.calc_angles(v[0], v[1], v[2], float(np.linalg.norm(v)), list(p.rotmatrix))
+        return math.acos(max(-1.0, min(1.0, ct))), ph
+
+    th, ph = ang([-8.0, 0.0, 0.0])
+    assert abs(th - math.pi / 2) < 1e-12 and abs(ph - math.pi) < 1e-12
+    th, ph = ang([0.0, 0.0, 5.0])
+    assert abs(th - math.pi / 2) < 1e-12 and abs(ph - math.pi / 2) < 1e-12
+    th, _ = ang([0.0, 5.0, 0.0])
+    assert abs(th - math.pi) < 1e-7
+    th, ph = ang([0.0, -3.0, 0.0])
+    assert abs(th) < 1e-7
+
+
+def test_calc_angles_is_a_rotation(orc):
+    """Angles from calc_angles reproduce the rotated direction with the
+    phi - pi shift: v = (-t_x, -t_y, t_z)/d, t = R^T (pos - opos)."""
+    p = orc.params_from_ini(os.path.join(GOLD, "synth_offaxis.ini"))
+    R = np.array(list(p.rotmatrix)).reshape(3, 3)
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        v = rng.standard_normal(3) * rng.uniform(0.1, 100)
+        d = float(np.linalg.norm(v))
+        ct, ph = orc.calc_angles(v[0], v[1], v[2], d, list(p.rotmatrix))
+        st = math.sqrt(max(0.0, 1 - ct * ct))
+        got = np.array([st * math.cos(ph), st * math.sin(ph), ct])
+        t = R.T @ v / d
+        assert np.allclose(got, [-t[0], -t[1], t[2]], atol=1e-12)
+        assert 0.0 <= ph <= 2 * math.pi

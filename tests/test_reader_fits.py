@@ -1,0 +1,81 @@
+"""Particle-file reader and FITS output.
+
+Reader: the oracle's native-format reader against records that the
+reference's own DataReader (src/datareader.cpp) returned for the same file
+(tests/golden/reader_kat.npz): chunked buffers, `test` truncation, mask.
+FITS: the library's HEALPix FITS writer against the FITS standard (2880-byte
+blocks, header keywords, big-endian float64 column) and a round trip.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def kat():
+    return np.load(os.path.join(GOLD, "reader_kat.npz"))
+
+
+def _write_native(path, parts, maskpath=None, mask=None):
+    with open(path, "wb") as fh:
+        np.array([len(parts)], dtype="<i4").tofile(fh)
+        np.ascontiguousarray(parts, dtype=np.float32).tofile(fh)
+    if maskpath is not None:
+        with open(maskpath, "wb") as fh:
+            np.array([len(mask)], dtype="<i4").tofile(fh)
+            np.asarray(mask, dtype=np.uint8).tofile(fh)
+
+
+@pytest.mark.parametrize("case", ["all_chunk97", "test333_chunk50", "mask_chunk128", "mask_test700_chunk1000"])
+def test_oracle_reader_matches_reference_datareader(orc, kat, tmp_path, case):
+    test, membuf, use_mask = (int(x) for x in kat[f"{case}__cfg"])
+    f = str(tmp_path / "p.bin")
+    m = str(tmp_path / "m.bin")
+    _write_native(f, kat["particles"], m, kat["mask"])
+    got = orc.read_native(f, test, m if use_mask else None)
+    want = kat[f"{case}__out"]
+    assert np.array_equal(got.view(np.float32).reshape(-1, 12), want)
+
+
+def test_reader_errors(orc, tmp_path):
+    with pytest.raises(IOError):
+        orc.read_native(str(tmp_path / "missing.bin"))
+    f = str(tmp_path / "p.bin")
+    m = str(tmp_path / "m.bin")
+    parts = np.zeros((10, 12), np.float32)
+    _write_native(f, parts, m, np.ones(9, np.uint8))  # count mismatch
+    with pytest.raises(IOError):
+        orc.read_native(f, -1, m)
+
+
+def test_fits_layout_and_round_trip(pw, tmp_path):
+    nside = 16
+    npix = 12 * nside * nside
+    rng = np.random.default_rng(0)
+    m = rng.standard_normal(npix)
+    path = str(tmp_path / "map.fits")
+    pw.write_fits(path, m, nside)
+    raw = open(path, "rb").read()
+    assert len(raw) % 2880 == 0
+    cards = [raw[i:i + 80].decode() for i in range(0, 2880, 80)]
+    assert cards[0].startswith("SIMPLE  =                    T")
+    ext = [raw[2880 + i:2880 + i + 80].decode() for i in range(0, 2880, 80)]
+    kv = {c[:8].strip(): c[10:].split("/")[0].strip() for c in ext if "=" in c[:10]}
+    assert kv["XTENSION"] == "'BINTABLE'"
+    assert kv["NAXIS1"] == str(8 * 1024) and kv["NAXIS2"] == str(npix // 1024)
+    assert kv["TFORM1"] == "'1024D   '"
+    assert kv["ORDERING"] == "'RING    '" and kv["PIXTYPE"] == "'HEALPIX '"
+    assert kv["NSIDE"] == str(nside) and kv["LASTPIX"] == str(npix - 1)
+    data = np.frombuffer(raw[5760:5760 + 8 * npix], dtype=">f8")
+    assert np.array_equal(data, m)
+    back, ns = pw.read_fits(path)
+    assert ns == nside and np.array_equal(back, m)
+    # small maps use one value per row
+    pw.write_fits(path, m[:48], 2)
+    back, ns = pw.read_fits(path)
+    assert ns == 2 and np.array_equal(back, m[:48])
+    assert struct.unpack(">d", open(path, "rb").read()[5760:5768])[0] == m[0]
