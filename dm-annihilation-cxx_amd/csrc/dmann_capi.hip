@@ -182,6 +182,19 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
     return DMANN_OK;
 }
 
+int dmann_adjoint_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n, const double *dev_grad_map,
+                         double *dev_grad_parts)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (n < 0) return fail(c, DMANN_ERR_ARG, "negative particle count");
+    if (n == 0) return DMANN_OK;
+    if (!dev_parts || !dev_grad_map || !dev_grad_parts) return fail(c, DMANN_ERR_ARG, "null pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_adjoint(reinterpret_cast<const Particle *>(dev_parts), n, c->P, dev_grad_map, dev_grad_parts,
+                             c->cfg.medium_blocks, c->stream));
+    return DMANN_OK;
+}
+
 int dmann_timing(dmann_ctx *c, int enable, double *ms, int *calls)
 {
     if (!c) return DMANN_ERR_ARG;

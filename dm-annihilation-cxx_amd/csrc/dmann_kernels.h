@@ -70,6 +70,8 @@ struct LaunchCfg {
 hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                          unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
                          hipStream_t st, hipEvent_t *ev = nullptr);
+hipError_t launch_adjoint(const Particle *d_parts, int64_t n, const DevParams &P, const double *gmap, double *out,
+                          int blocks, hipStream_t st);
 hipError_t launch_finalize(int acc, MapRef map, int64_t npix, double dOmega, double *out, hipStream_t st);
 hipError_t launch_clear(int acc, MapRef map, int64_t npix, unsigned long long *stats, hipStream_t st);
 

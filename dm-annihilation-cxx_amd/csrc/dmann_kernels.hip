@@ -445,6 +445,135 @@ __global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ 
     }
 }
 
+
+// ---------------------------------------------------------------- adjoint
+
+// Gradient of a loss L(map) with respect to each particle's mass, density,
+// smoothing length and position, given g = dL/dmap (FP64, the finished map,
+// i.e. after the division by dOmega).  The footprint is recomputed with the
+// forward's ring walk (membership is piecewise constant, so only the smooth
+// dependence through flux, r and the direction is differentiated), weights
+// and their derivatives in FP64:
+//   w_p = exp(-c psi_p^2 / r^2), c = 0.5/0.333,
+//   dw_p/dr = w_p 2 c psi_p^2 / r^3,
+//   dw_p/dv = w_p 2 c psi_p / (r^2 sin psi_p) (u_p - cos psi_p v)
+// with v the particle direction and u_p the pixel centre.  Per particle the
+// wave sums S = sum g w, N = sum w and the r- and v-derivatives of both;
+// then dL/dflux = S/N, dL/dr = flux (S_r N - S N_r)/N^2,
+// dL/dv = flux (S_v N - S N_v)/N^2, chained to (m, rho, h, pos) through
+// flux = K rho m / (4 pi d^2), r = h/d, v = diag(-1,-1,1) R^T (pos-opos)/d.
+// One wavefront per particle; out[6 i + {0..5}] = d/d(mass, dens, hsmooth,
+// posx, posy, posz).
+__global__ __launch_bounds__(256) void k_adjoint(const Particle *__restrict__ parts, int64_t n, DevParams P,
+                                                 const double *__restrict__ gmap, double *__restrict__ out)
+{
+    __shared__ WinLDS lds[4];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    WinLDS &L = lds[wid];
+    const double c = 0.5 / 0.333;
+    const double inv_dO = 1.0 / P.dOmega;
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wid; i < n; i += (int64_t)gridDim.x * 4) {
+        const Particle q = parts[i];
+        DiscRec rec;
+        bool bad;
+        const int cls = prepass_one(P, q, rec, bad);
+        double res[6] = {0, 0, 0, 0, 0, 0};
+        if (cls != kClsSkip) {
+            const double dx = (double)q.posx - P.opos[0];
+            const double dy = (double)q.posy - P.opos[1];
+            const double dz = (double)q.posz - P.opos[2];
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            const double flux = rec.flux;
+            const double h = (double)q.hsmooth;
+            double S = 0, N = 0, Sr = 0, Nr = 0, Sv[3] = {0, 0, 0}, Nv[3] = {0, 0, 0};
+            int T = 0;
+            double r = rec.r;
+            if (cls != kClsPoint) {
+                const PGeo g = particle_geo(rec, P.nside);
+                const double st = sin(g.theta);
+                const double v[3] = {st * cos(g.phi), st * sin(g.phi), g.z0};
+                const int nwin = (g.nrings + 63) / 64;
+                for (int win = 0; win < nwin; ++win) {
+                    const int Tw = setup_window<true>(g, P.nside, win, L, lane, 0);
+                    int j = 0;
+                    for (int k = lane; k < Tw; k += 64) {
+                        while (L.pre_end[j] <= k) ++j;
+                        const double4 dd = L.d[j];
+                        const int4 a = L.a[j];
+                        const double phic = ((double)(k - a.w) + dd.z) * dd.w;
+                        double sp, cp;
+                        sincos(phic, &sp, &cp);
+                        const double u[3] = {dd.y * cp, dd.y * sp, dd.x};
+                        const double cosp = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+                        const double cx = u[1] * v[2] - u[2] * v[1];
+                        const double cy = u[2] * v[0] - u[0] * v[2];
+                        const double cz = u[0] * v[1] - u[1] * v[0];
+                        const double sinp = sqrt(cx * cx + cy * cy + cz * cz);
+                        const double psi = atan2(sinp, cosp);
+                        const double w = exp(-c * psi * psi / (r * r));
+                        const double gp = gmap[pix_addr(a, k)] * inv_dO;
+                        const double dwdr = w * 2.0 * c * psi * psi / (r * r * r);
+                        const double ratio = sinp > 1e-300 ? psi / sinp : 1.0;
+                        const double fv = w * 2.0 * c * ratio / (r * r);
+                        S += gp * w;
+                        N += w;
+                        Sr += gp * dwdr;
+                        Nr += dwdr;
+#pragma unroll
+                        for (int e = 0; e < 3; e++) {
+                            const double dv = fv * (u[e] - cosp * v[e]);
+                            Sv[e] += gp * dv;
+                            Nv[e] += dv;
+                        }
+                    }
+                    T += Tw;
+                    wave_sync();
+                }
+                S = wave_sum(S);
+                N = wave_sum(N);
+                Sr = wave_sum(Sr);
+                Nr = wave_sum(Nr);
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    Sv[e] = wave_sum(Sv[e]);
+                    Nv[e] = wave_sum(Nv[e]);
+                }
+            }
+            double dLdflux, dLdr = 0.0, dLdv[3] = {0, 0, 0};
+            if (cls == kClsPoint || T < 2) {
+                dLdflux = gmap[rec.p0] * inv_dO;
+            } else {
+                dLdflux = S / N;
+                dLdr = flux * (Sr * N - S * Nr) / (N * N);
+                for (int e = 0; e < 3; e++) dLdv[e] = flux * (Sv[e] * N - S * Nv[e]) / (N * N);
+            }
+            const double inv4pid2 = 1.0 / (4.0 * kPi * d * d);
+            res[0] = dLdflux * P.flux_factor * (double)q.dens * inv4pid2;
+            res[1] = dLdflux * P.flux_factor * (double)q.mass * inv4pid2;
+            res[2] = dLdr / d;
+            const double dLdd = dLdflux * (-2.0 * flux / d) + dLdr * (-h / (d * d));
+            // R D dL/dv, D = diag(-1, -1, 1); R row-major
+            const double wv[3] = {-dLdv[0], -dLdv[1], dLdv[2]};
+            double rw[3];
+            for (int a = 0; a < 3; a++) rw[a] = P.rot[3 * a] * wv[0] + P.rot[3 * a + 1] * wv[1] + P.rot[3 * a + 2] * wv[2];
+            const double dh[3] = {dx / d, dy / d, dz / d};
+            const double proj = rw[0] * dh[0] + rw[1] * dh[1] + rw[2] * dh[2];
+            for (int a = 0; a < 3; a++) res[3 + a] = (rw[a] - proj * dh[a]) / d + dLdd * dh[a];
+        }
+        if (lane < 6) out[6 * i + lane] = res[lane];
+    }
+}
+
+hipError_t launch_adjoint(const Particle *d_parts, int64_t n, const DevParams &P, const double *gmap, double *out,
+                          int blocks, hipStream_t st)
+{
+    const int64_t want = (n + 3) / 4;
+    const int b = (int)(want < blocks ? want : blocks);
+    if (b > 0) k_adjoint<<<b, 256, 0, st>>>(d_parts, n, P, gmap, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- map ops
 
 __global__ void k_fill_f32(float *p, int64_t n)

@@ -50,7 +50,7 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_set_stream", "dmann_synchronize", "dmann_get_map", "dmann_get_map_device",
            "dmann_reset", "dmann_get_stats", "dmann_npix", "dmann_destroy",
            "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint",
-           "dmann_timing", "dmann_last_error")
+           "dmann_timing", "dmann_adjoint_device", "dmann_last_error")
 
 _lib = None
 
@@ -90,6 +90,7 @@ def load_library():
     L.dmann_host_footprint.argtypes = [C.POINTER(DmannParams), vp, vp, vp, i64, C.POINTER(C.c_double),
                                        C.POINTER(C.c_int)]
     L.dmann_host_footprint.restype = i64
+    L.dmann_adjoint_device.argtypes = [vp, vp, i64, vp, vp]
     L.dmann_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.dmann_last_error.argtypes = [vp]
     L.dmann_last_error.restype = C.c_char_p
@@ -211,6 +212,24 @@ class Projector:
             raise ValueError("out must be a float64 device tensor of npix elements")
         self._bind_stream()
         _check(self._L.dmann_get_map_device(self._h, C.c_void_p(out.data_ptr())), self._h)
+        return out
+
+    def adjoint(self, parts, grad_map, out=None):
+        """dL/d(mass, dens, hsmooth, posx, posy, posz) per particle (FP64,
+        (n, 6) device tensor) for dL/dmap `grad_map` (float64 device tensor of
+        npix).  `parts` is a (n, 12) float32 device tensor."""
+        import torch
+        n = parts.shape[0]
+        if out is None:
+            out = torch.empty(n, 6, dtype=torch.float64, device=parts.device)
+        g = grad_map.contiguous()
+        if g.dtype != torch.float64 or g.numel() != self.npix or not g.is_cuda:
+            raise ValueError("grad_map must be a float64 device tensor of npix elements")
+        if not (parts.is_cuda and parts.dtype == torch.float32 and parts.is_contiguous()):
+            raise ValueError("parts must be a contiguous float32 device tensor")
+        self._bind_stream()
+        _check(self._L.dmann_adjoint_device(self._h, C.c_void_p(parts.data_ptr()), int(n),
+                                            C.c_void_p(g.data_ptr()), C.c_void_p(out.data_ptr())), self._h)
         return out
 
     def timing(self, enable: bool):

@@ -122,6 +122,17 @@ int dmann_set_stream(dmann_ctx *ctx, void *stream);
  * particle had an undefined direction. */
 int dmann_synchronize(dmann_ctx *ctx);
 
+/* Adjoint of the map with respect to the particles (no counterpart in the
+ * reference, which has no gradients).  Given dev_grad_map = dL/dmap for the
+ * finished map (npix doubles on the device, the map after division by
+ * dOmega), writes dev_grad_parts[6*i + {0..5}] = dL/d(mass, dens, hsmooth,
+ * posx, posy, posz) of particle i (FP64, device).  The footprint is recomputed
+ * with the forward's ring walk; weights and derivatives are FP64.  Pixel
+ * membership of the footprint is piecewise constant: its jumps are not part of
+ * the gradient.  Asynchronous on the context's stream. */
+int dmann_adjoint_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t n, const double *dev_grad_map,
+                         double *dev_grad_parts);
+
 /* Per-kernel timing with HIP events on the context's stream.  Reads the
  * summed durations (ms) of the kernels of the render calls made since the
  * last call -- ms[0] pre-pass, ms[1] sky-covering kernel, ms[2] medium
