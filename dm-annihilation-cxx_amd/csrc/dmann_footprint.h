@@ -11,6 +11,12 @@
 
 namespace dmann {
 
+// The fast weight form (MODE 0 of weight32) holds when the footprint radius
+// rad = 2r is at most 0.5 and its longitude half-width
+// asin(sin(rad)/sin(theta)) stays below 0.445; this also keeps the disc off
+// both poles.  s0 = sin(theta) as sqrt((1 - cos theta)(1 + cos theta)).
+DMANN_HD bool fast_geometry(double rad, double s0) { return rad <= 0.5 && sin(rad) < s0 * 0.43; }
+
 // Pre-pass of one particle (src/genmap.cpp:82-110; src/anglefuncs.cpp:15-71;
 // src/flux/flux_ann_nosom.cpp:12-23).  Returns kClsSkip (filtered),
 // kClsPoint (whole flux to rec.p0), kClsMedium or kClsLarge (footprint).
@@ -56,7 +62,11 @@ DMANN_HD int prepass_one(const DevParams &P, const Particle &q, DiscRec &rec, bo
     const double rad = 2.0 * r;
     const double est = (rad >= kPi) ? (double)P.npix : kTwoPi * (1.0 - cos(rad)) / P.dOmega;
     rec.est = est > 2.0e9 ? 2000000000 : (int)est;
-    rec.cls = (rad >= kPi || est > (double)P.large_pix) ? kClsLarge : kClsMedium;
+    const double z0 = cos(theta);
+    const double s0 = sqrt((1 - z0) * (1 + z0));
+    // medium: one wavefront, fast weight form; large: everything else
+    // (sky-covering, polar, wide) on the workgroup kernel
+    rec.cls = (fast_geometry(rad, s0) && est <= (double)P.large_pix) ? kClsMedium : kClsLarge;
     return rec.cls;
 }
 
@@ -97,10 +107,7 @@ DMANN_HD PGeo particle_geo(const DiscRec &rec, int nside)
     g.nrings = g.all ? 4 * nside - 1 : g.nnorth + g.npart + g.nsouth;
     const double kc = 0.5 / 0.333; // src/genmap.cpp:143
     g.K4 = (float)(-kc * 4.0 * 1.4426950408889634 / (g.r * g.r));
-    // the fast polynomials hold for psi <= 0.5 and |dphi| <= 0.5
-    bool fast = !g.all && !north && !south && g.rad <= 0.5;
-    if (fast) fast = sin(g.rad) < g.s0 * 0.43; // max |dphi| = asin(sin R / sin theta) < 0.445
-    g.mode = fast ? 0 : 1;
+    g.mode = fast_geometry(g.rad, g.s0) ? 0 : 1;
     return g;
 }
 

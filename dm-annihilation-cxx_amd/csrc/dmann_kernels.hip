@@ -345,9 +345,8 @@ __global__ __launch_bounds__(256) void k_disc_medium(const DiscRec *__restrict__
     unsigned long long n_ext = 0, n_fb = 0, n_contrib = 0;
     for (unsigned it = blockIdx.x * 4 + wid; it < count; it += nwaves) {
         const PGeo g = particle_geo(recs[it], nside);
-        int T;
-        if (g.mode == 0) T = render_particle_wave<ACC, 0>(g, nside, L, lane, map);
-        else T = render_particle_wave<ACC, 1>(g, nside, L, lane, map);
+        // the pre-pass only puts fast-form particles in this class
+        const int T = render_particle_wave<ACC, 0>(g, nside, L, lane, map);
         if (T < 2) n_fb++;
         else {
             n_ext++;
