@@ -1,0 +1,280 @@
+// Batched maps: many observers (each with its own nside) rendered from one
+// particle set in one persistent launch, the whole render -- map clears,
+// the persistent kernel, the FP64 finalisation of every map -- captured once
+// in a hipGraph and replayed.  No counterpart in the reference, which renders
+// one observer per run.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <new>
+#include <vector>
+
+#include "dmann.h"
+#include "dmann_geom.h"
+#include "dmann_kernels.h"
+
+using namespace dmann;
+
+struct dmann_batch {
+    int device = 0;
+    int acc = kAccF32;
+    int nmaps = 0;
+    hipStream_t stream = nullptr;
+    std::vector<DevParams> P;
+    std::vector<MapRef> maps;
+    std::vector<double *> out;
+    std::vector<int64_t> npix;
+    DevParams *d_P = nullptr;
+    MapRef *d_maps = nullptr;
+    unsigned long long *d_work = nullptr;
+    unsigned long long *d_stats = nullptr;
+    int chunk = 64;
+    int blocks = 0;
+    int cus = 256;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *graph_parts = nullptr;
+    int64_t graph_n = -1;
+    long long replays = 0;
+    char msg[512] = {0};
+};
+
+static char g_batch_msg[512] = "";
+
+static int bfail(dmann_batch *b, int code, const char *fmt, ...)
+{
+    char *dst = b ? b->msg : g_batch_msg;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(dst, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define BCHK(b, call)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return bfail((b), DMANN_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static void drop_graph(dmann_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_parts = nullptr;
+    b->graph_n = -1;
+}
+
+static void batch_free(dmann_batch *b)
+{
+    drop_graph(b);
+    for (auto &m : b->maps) {
+        (void)hipFree(m.f32);
+        (void)hipFree(m.lo);
+        (void)hipFree(m.f64);
+    }
+    for (double *o : b->out) (void)hipFree(o);
+    (void)hipFree(b->d_P);
+    (void)hipFree(b->d_maps);
+    (void)hipFree(b->d_work);
+    (void)hipFree(b->d_stats);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    b->maps.clear();
+    b->out.clear();
+}
+
+// Enqueues one full render of the batch on its stream.
+static hipError_t enqueue(dmann_batch *b, const Particle *parts, int64_t n)
+{
+    hipError_t e;
+    for (int m = 0; m < b->nmaps; m++)
+        if ((e = launch_clear(b->acc, b->maps[m], b->npix[m], b->d_stats + (size_t)m * kNumStats, b->stream)) !=
+            hipSuccess)
+            return e;
+    if (n > 0 &&
+        (e = launch_batch(b->acc, parts, n, b->d_P, b->d_maps, b->nmaps, b->chunk, b->d_work, b->d_stats, b->blocks,
+                          b->stream)) != hipSuccess)
+        return e;
+    for (int m = 0; m < b->nmaps; m++)
+        if ((e = launch_finalize(b->acc, b->maps[m], b->npix[m], b->P[m].dOmega, b->out[m], b->stream)) != hipSuccess)
+            return e;
+    return hipSuccess;
+}
+
+extern "C" {
+
+int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps, int device)
+{
+    if (!out || !params || nmaps < 1) return bfail(nullptr, DMANN_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    const int prec = params[0].precision;
+    if (prec < DMANN_PREC_FP32 || prec > DMANN_PREC_FP64) return bfail(nullptr, DMANN_ERR_ARG, "unknown precision");
+    for (int m = 0; m < nmaps; m++) {
+        if (params[m].nside < 1 || params[m].nside > kMaxNside)
+            return bfail(nullptr, DMANN_ERR_ARG, "map %d: nside %d outside 1..%d", m, params[m].nside, kMaxNside);
+        if (params[m].precision != prec)
+            return bfail(nullptr, DMANN_ERR_ARG, "map %d: all maps of a batch share one precision", m);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(nullptr, DMANN_ERR_HIP, "no HIP device");
+    if (device < 0 || device >= ndev) return bfail(nullptr, DMANN_ERR_ARG, "device %d of %d", device, ndev);
+    dmann_batch *b = new (std::nothrow) dmann_batch();
+    if (!b) return bfail(nullptr, DMANN_ERR_ARG, "out of host memory");
+    b->device = device;
+    b->acc = prec == DMANN_PREC_FP32 ? kAccF32 : prec == DMANN_PREC_COMPENSATED ? kAccComp : kAccF64;
+    b->nmaps = nmaps;
+    auto bail = [&](const char *what, hipError_t err) {
+        snprintf(g_batch_msg, sizeof g_batch_msg, "%s: %s", what, hipGetErrorString(err));
+        batch_free(b);
+        delete b;
+        return DMANN_ERR_HIP;
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return bail("hipSetDevice", e);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        b->cus = prop.multiProcessorCount;
+    b->blocks = b->cus * 8;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
+    for (int m = 0; m < nmaps; m++) {
+        const dmann_params &p = params[m];
+        DevParams P{};
+        for (int i = 0; i < 3; i++) P.opos[i] = p.opos[i];
+        for (int i = 0; i < 9; i++) P.rot[i] = p.rotmatrix[i];
+        P.flux_factor = p.flux_factor;
+        P.mass_cut = p.mass_cut;
+        P.nside = p.nside;
+        P.npix = (int)npix_of(p.nside);
+        P.dOmega = 4 * kPi / (double)P.npix;
+        P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592));
+        P.large_pix = 1 << 16;
+        b->P.push_back(P);
+        b->npix.push_back(P.npix);
+        MapRef r{nullptr, nullptr, nullptr};
+        double *o = nullptr;
+        b->maps.push_back(r);
+        b->out.push_back(o);
+        MapRef &mr = b->maps.back();
+        if (b->acc == kAccF64) {
+            if ((e = hipMalloc(&mr.f64, P.npix * sizeof(double))) != hipSuccess) return bail("hipMalloc map", e);
+        } else {
+            if ((e = hipMalloc(&mr.f32, P.npix * sizeof(float))) != hipSuccess) return bail("hipMalloc map", e);
+            if (b->acc == kAccComp && (e = hipMalloc(&mr.lo, P.npix * sizeof(float))) != hipSuccess)
+                return bail("hipMalloc map lo", e);
+        }
+        if ((e = hipMalloc(&b->out.back(), P.npix * sizeof(double))) != hipSuccess) return bail("hipMalloc out", e);
+    }
+    if ((e = hipMalloc(&b->d_P, nmaps * sizeof(DevParams))) != hipSuccess) return bail("hipMalloc params", e);
+    if ((e = hipMalloc(&b->d_maps, nmaps * sizeof(MapRef))) != hipSuccess) return bail("hipMalloc maps", e);
+    if ((e = hipMalloc(&b->d_work, sizeof(unsigned long long))) != hipSuccess) return bail("hipMalloc work", e);
+    if ((e = hipMalloc(&b->d_stats, (size_t)nmaps * kNumStats * sizeof(unsigned long long))) != hipSuccess)
+        return bail("hipMalloc stats", e);
+    if ((e = hipMemcpy(b->d_P, b->P.data(), nmaps * sizeof(DevParams), hipMemcpyHostToDevice)) != hipSuccess)
+        return bail("copy params", e);
+    if ((e = hipMemcpy(b->d_maps, b->maps.data(), nmaps * sizeof(MapRef), hipMemcpyHostToDevice)) != hipSuccess)
+        return bail("copy maps", e);
+    *out = b;
+    return DMANN_OK;
+}
+
+int dmann_batch_set_tuning(dmann_batch *b, int chunk, int blocks_per_cu)
+{
+    if (!b) return DMANN_ERR_ARG;
+    if (chunk < 1 || chunk > (1 << 20) || blocks_per_cu < 1 || blocks_per_cu > 64)
+        return bfail(b, DMANN_ERR_ARG, "tuning out of range (chunk 1..2^20, blocks per CU 1..64)");
+    b->chunk = chunk;
+    b->blocks = b->cus * blocks_per_cu;
+    drop_graph(b);
+    return DMANN_OK;
+}
+
+int dmann_batch_render_device(dmann_batch *b, const dmann_particle *dev_parts, int64_t n, int use_graph)
+{
+    if (!b) return DMANN_ERR_ARG;
+    if (n < 0 || (n > 0 && !dev_parts)) return bfail(b, DMANN_ERR_ARG, "bad particle arguments");
+    BCHK(b, hipSetDevice(b->device));
+    const Particle *parts = reinterpret_cast<const Particle *>(dev_parts);
+    if (!use_graph) {
+        BCHK(b, enqueue(b, parts, n));
+        return DMANN_OK;
+    }
+    if (!b->exec || b->graph_parts != (const void *)dev_parts || b->graph_n != n) {
+        drop_graph(b);
+        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue(b, parts, n);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return bfail(b, DMANN_ERR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        b->graph = g;
+        BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_parts = dev_parts;
+        b->graph_n = n;
+    }
+    BCHK(b, hipGraphLaunch(b->exec, b->stream));
+    b->replays++;
+    return DMANN_OK;
+}
+
+int dmann_batch_synchronize(dmann_batch *b)
+{
+    if (!b) return DMANN_ERR_ARG;
+    BCHK(b, hipSetDevice(b->device));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return DMANN_OK;
+}
+
+int64_t dmann_batch_npix(const dmann_batch *b, int i) { return (b && i >= 0 && i < b->nmaps) ? b->npix[i] : -1; }
+
+int dmann_batch_map_device(dmann_batch *b, int i, double **dev_map)
+{
+    if (!b || !dev_map || i < 0 || i >= b->nmaps) return bfail(b, DMANN_ERR_ARG, "bad map index");
+    *dev_map = b->out[i];
+    return DMANN_OK;
+}
+
+int dmann_batch_get_map(dmann_batch *b, int i, double *host_map)
+{
+    if (!b || !host_map || i < 0 || i >= b->nmaps) return bfail(b, DMANN_ERR_ARG, "bad map index");
+    BCHK(b, hipSetDevice(b->device));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    BCHK(b, hipMemcpy(host_map, b->out[i], b->npix[i] * sizeof(double), hipMemcpyDeviceToHost));
+    return DMANN_OK;
+}
+
+int dmann_batch_get_stats(dmann_batch *b, int i, dmann_stats *st)
+{
+    if (!b || !st || i < 0 || i >= b->nmaps) return bfail(b, DMANN_ERR_ARG, "bad map index");
+    BCHK(b, hipSetDevice(b->device));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    unsigned long long h[kNumStats];
+    BCHK(b, hipMemcpy(h, b->d_stats + (size_t)i * kNumStats, sizeof h, hipMemcpyDeviceToHost));
+    st->kept = (int64_t)h[kStatKept];
+    st->subpixel = (int64_t)h[kStatSub];
+    st->fallback = (int64_t)h[kStatFallback];
+    st->extended = (int64_t)h[kStatExt];
+    st->contributions = (int64_t)h[kStatContrib];
+    st->large = 0;
+    return DMANN_OK;
+}
+
+long long dmann_batch_graph_replays(const dmann_batch *b) { return b ? b->replays : -1; }
+
+void dmann_batch_destroy(dmann_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    delete b;
+}
+
+const char *dmann_batch_last_error(const dmann_batch *b) { return b ? b->msg : g_batch_msg; }
+
+} // extern "C"

@@ -33,6 +33,7 @@ struct dmann_ctx {
     MapRef map{nullptr, nullptr, nullptr};
     double *d_out = nullptr; // finalised FP64 map for host reads
     LaunchCfg cfg{};
+    int cus = 0;
     bool timing = false;
     std::vector<hipEvent_t> events; // 4 per timed render call
     char msg[512] = {0};
@@ -122,6 +123,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
             dev_cus = prop.multiProcessorCount;
     }
+    c->cus = dev_cus;
     c->cfg.prepass_blocks = dev_cus * 64;
     c->cfg.medium_blocks = dev_cus * 8;
     c->cfg.large_blocks = dev_cus * 4;
@@ -148,6 +150,20 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);
     *out = c;
+    return DMANN_OK;
+}
+
+int dmann_set_tuning(dmann_ctx *c, int medium_blocks_per_cu, int large_blocks_per_cu, int large_pix)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (medium_blocks_per_cu < 1 || medium_blocks_per_cu > 64 || large_blocks_per_cu < 1 ||
+        large_blocks_per_cu > 64 || large_pix < 16)
+        return fail(c, DMANN_ERR_ARG, "tuning out of range");
+    int cus = c->cfg.medium_blocks / 8 > 0 ? c->cfg.medium_blocks / 8 : 1;
+    if (c->cus > 0) cus = c->cus;
+    c->cfg.medium_blocks = cus * medium_blocks_per_cu;
+    c->cfg.large_blocks = cus * large_blocks_per_cu;
+    c->P.large_pix = large_pix;
     return DMANN_OK;
 }
 

@@ -72,6 +72,10 @@ hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevP
                          hipStream_t st, hipEvent_t *ev = nullptr);
 hipError_t launch_adjoint(const Particle *d_parts, int64_t n, const DevParams &P, const double *gmap, double *out,
                           int blocks, hipStream_t st);
+hipError_t launch_batch(int acc, const Particle *d_parts, int64_t n, const DevParams *Ps, const MapRef *maps,
+                        int nmaps, int chunk, unsigned long long *work, unsigned long long *stats, int blocks,
+                        hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t st);
 hipError_t launch_finalize(int acc, MapRef map, int64_t npix, double dOmega, double *out, hipStream_t st);
 hipError_t launch_clear(int acc, MapRef map, int64_t npix, unsigned long long *stats, hipStream_t st);
 

@@ -573,6 +573,96 @@ hipError_t launch_adjoint(const Particle *d_parts, int64_t n, const DevParams &P
     return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------- batched
+
+// Many maps (observers, each with its own nside) from one particle set in
+// one persistent launch.  Work items are (map, chunk of `chunk` particles),
+// taken from a device counter by whole wavefronts; for each particle the
+// wave runs the pre-pass for that map's observer and renders the footprint
+// (fast or general weight form).  Every wave leaves when the counter passes
+// the last item.
+template <int ACC>
+__global__ __launch_bounds__(256) void k_batch(const Particle *__restrict__ parts, int64_t n,
+                                               const DevParams *__restrict__ Ps, const MapRef *__restrict__ maps,
+                                               int nmaps, int chunk, unsigned long long *__restrict__ work,
+                                               unsigned long long *__restrict__ stats)
+{
+    __shared__ WinLDS lds[4];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    WinLDS &L = lds[wid];
+    const unsigned long long nblk = (unsigned long long)((n + chunk - 1) / chunk);
+    const unsigned long long total = nblk * (unsigned long long)nmaps;
+    for (;;) {
+        unsigned long long item = 0;
+        if (lane == 0) item = atomicAdd(work, 1ull);
+        item = __shfl(item, 0, 64);
+        if (item >= total) break;
+        const int m = (int)(item / nblk);
+        const int64_t i0 = (int64_t)(item % nblk) * chunk;
+        const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+        const DevParams P = Ps[m];
+        const MapRef map = maps[m];
+        unsigned long long c_kept = 0, c_sub = 0, c_fb = 0, c_ext = 0, c_contrib = 0;
+        for (int64_t i = i0; i < i1; ++i) {
+            const Particle q = parts[i];
+            DiscRec rec;
+            bool bad;
+            const int cls = prepass_one(P, q, rec, bad);
+            if (cls == kClsSkip) continue;
+            c_kept++;
+            if (cls == kClsPoint) {
+                if (lane == 0) acc_point<ACC>(map, rec.p0, rec.flux);
+                c_sub++;
+                continue;
+            }
+            const PGeo g = particle_geo(rec, P.nside);
+            const int T = g.mode == 0 ? render_particle_wave<ACC, 0>(g, P.nside, L, lane, map)
+                                      : render_particle_wave<ACC, 1>(g, P.nside, L, lane, map);
+            if (T < 2) c_fb++;
+            else {
+                c_ext++;
+                c_contrib += (unsigned long long)T;
+            }
+        }
+        if (lane == 0) {
+            unsigned long long *st = stats + (size_t)m * kNumStats;
+            if (c_kept) atomicAdd(&st[kStatKept], c_kept);
+            if (c_sub) atomicAdd(&st[kStatSub], c_sub);
+            if (c_fb) atomicAdd(&st[kStatFallback], c_fb);
+            if (c_ext) atomicAdd(&st[kStatExt], c_ext);
+            if (c_contrib) atomicAdd(&st[kStatContrib], c_contrib);
+        }
+    }
+}
+
+__global__ void k_fill_u64n(unsigned long long *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+hipError_t launch_batch(int acc, const Particle *d_parts, int64_t n, const DevParams *Ps, const MapRef *maps,
+                        int nmaps, int chunk, unsigned long long *work, unsigned long long *stats, int blocks,
+                        hipStream_t st)
+{
+    k_fill_u64n<<<1, 64, 0, st>>>(work, 1);
+    switch (acc) {
+    case kAccF32: k_batch<kAccF32><<<blocks, 256, 0, st>>>(d_parts, n, Ps, maps, nmaps, chunk, work, stats); break;
+    case kAccComp: k_batch<kAccComp><<<blocks, 256, 0, st>>>(d_parts, n, Ps, maps, nmaps, chunk, work, stats); break;
+    default: k_batch<kAccF64><<<blocks, 256, 0, st>>>(d_parts, n, Ps, maps, nmaps, chunk, work, stats); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t st)
+{
+    const int blocks = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+    k_fill_u64n<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- map ops
 
 __global__ void k_fill_f32(float *p, int64_t n)

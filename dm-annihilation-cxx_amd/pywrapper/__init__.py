@@ -5,9 +5,10 @@ this package exposes the map path itself:
   capi      ctypes binding of libdmann.so (Projector, params, FITS I/O)
   synth     seeded synthetic NFW / Plummer particle sets
   autograd  torch.autograd.Function over the projector and its adjoint
+  autotune  launch-shape / size-class autotuner
 """
-from . import autograd, capi, synth  # noqa: F401
+from . import autograd, autotune, capi, synth  # noqa: F401
 from .autograd import ProjectMap, project_map  # noqa: F401
-from .capi import (PARTICLE_DTYPE, PREC_COMPENSATED, PREC_FP32, PREC_FP64, DmannError,  # noqa: F401
+from .capi import (PARTICLE_DTYPE, PREC_COMPENSATED, PREC_FP32, PREC_FP64, Batch, DmannError,  # noqa: F401
                    DmannParams, Projector, load_library, params_for_observer, params_from_ini,
                    read_fits, write_fits, write_ini)

@@ -1,0 +1,54 @@
+"""Autotuner for the projector's launch shapes and size-class threshold.
+
+Times candidate settings on a particle sample with HIP events (through
+torch's stream) and keeps the fastest.  Single map: persistent workgroups
+per CU of the two footprint kernels and the footprint size that sends a
+particle to the workgroup kernel.  Batch: particles per work item and
+workgroups per CU of the persistent kernel.
+"""
+from __future__ import annotations
+
+import itertools
+
+import torch
+
+
+def _time(fn, reps: int = 2) -> float:
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune_projector(projector, parts, medium=(4, 8, 16), large=(2, 4, 8), large_pix=(16384, 65536, 262144),
+                   reps: int = 2):
+    """Returns (best settings dict, {settings: ms}) and applies the best."""
+    results = {}
+
+    def run():
+        projector.reset()
+        projector.render(parts)
+
+    for m, l, lp in itertools.product(medium, large, large_pix):
+        projector.set_tuning(m, l, lp)
+        results[(m, l, lp)] = _time(run, reps)
+    best = min(results, key=results.get)
+    projector.set_tuning(*best)
+    projector.reset()
+    return {"medium_blocks_per_cu": best[0], "large_blocks_per_cu": best[1], "large_pix": best[2]}, results
+
+
+def tune_batch(batch, parts, chunks=(16, 64, 256), blocks=(4, 8, 16), reps: int = 2):
+    results = {}
+    for c, b in itertools.product(chunks, blocks):
+        batch.set_tuning(c, b)
+        results[(c, b)] = _time(lambda: batch.render(parts, use_graph=True), reps)
+    best = min(results, key=results.get)
+    batch.set_tuning(*best)
+    return {"chunk": best[0], "blocks_per_cu": best[1]}, results

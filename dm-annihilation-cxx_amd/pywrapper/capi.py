@@ -50,7 +50,11 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_set_stream", "dmann_synchronize", "dmann_get_map", "dmann_get_map_device",
            "dmann_reset", "dmann_get_stats", "dmann_npix", "dmann_destroy",
            "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint",
-           "dmann_timing", "dmann_adjoint_device", "dmann_last_error")
+           "dmann_timing", "dmann_adjoint_device", "dmann_last_error", "dmann_set_tuning",
+           "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device",
+           "dmann_batch_synchronize", "dmann_batch_npix", "dmann_batch_map_device", "dmann_batch_get_map",
+           "dmann_batch_get_stats", "dmann_batch_graph_replays", "dmann_batch_destroy",
+           "dmann_batch_last_error")
 
 _lib = None
 
@@ -91,9 +95,25 @@ def load_library():
                                        C.POINTER(C.c_int)]
     L.dmann_host_footprint.restype = i64
     L.dmann_adjoint_device.argtypes = [vp, vp, i64, vp, vp]
+    L.dmann_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.dmann_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.dmann_last_error.argtypes = [vp]
     L.dmann_last_error.restype = C.c_char_p
+    L.dmann_batch_create.argtypes = [C.POINTER(vp), vp, C.c_int, C.c_int]
+    L.dmann_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.dmann_batch_render_device.argtypes = [vp, vp, i64, C.c_int]
+    L.dmann_batch_synchronize.argtypes = [vp]
+    L.dmann_batch_npix.argtypes = [vp, C.c_int]
+    L.dmann_batch_npix.restype = i64
+    L.dmann_batch_map_device.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.dmann_batch_get_map.argtypes = [vp, C.c_int, vp]
+    L.dmann_batch_get_stats.argtypes = [vp, C.c_int, C.POINTER(DmannStats)]
+    L.dmann_batch_graph_replays.argtypes = [vp]
+    L.dmann_batch_graph_replays.restype = C.c_longlong
+    L.dmann_batch_destroy.argtypes = [vp]
+    L.dmann_batch_destroy.restype = None
+    L.dmann_batch_last_error.argtypes = [vp]
+    L.dmann_batch_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -232,6 +252,10 @@ class Projector:
                                             C.c_void_p(g.data_ptr()), C.c_void_p(out.data_ptr())), self._h)
         return out
 
+    def set_tuning(self, medium_blocks_per_cu: int = 8, large_blocks_per_cu: int = 4, large_pix: int = 65536):
+        _check(self._L.dmann_set_tuning(self._h, int(medium_blocks_per_cu), int(large_blocks_per_cu),
+                                        int(large_pix)), self._h)
+
     def timing(self, enable: bool):
         """Returns ({'prepass','large','medium'}: summed ms, calls) of the
         render calls since the last timing() call, then enables/disables
@@ -287,6 +311,72 @@ def host_footprint(params: DmannParams, rec):
             m = 1 if kind.value in (1, 2) else int(n)
             return kind.value, pix[:m].copy(), w[:m].copy(), fl.value
         cap = int(n)
+
+
+# ------------------------------------------------------------------ batch
+
+class Batch:
+    """Many maps (observers, mixed nside) of one particle set, rendered in
+    one persistent launch replayed from a hipGraph."""
+
+    def __init__(self, params_list, device: int = 0):
+        L = load_library()
+        self._L = L
+        arr = (DmannParams * len(params_list))(*[p.copy() for p in params_list])
+        self._params = arr
+        h = C.c_void_p()
+        code = L.dmann_batch_create(C.byref(h), C.cast(arr, C.c_void_p), len(params_list), int(device))
+        if code != 0:
+            raise DmannError(code, (L.dmann_batch_last_error(None) or b"").decode())
+        self._h = h
+        self.nmaps = len(params_list)
+        self.npix = [int(L.dmann_batch_npix(h, i)) for i in range(self.nmaps)]
+
+    def _chk(self, code):
+        if code != 0:
+            raise DmannError(code, (self._L.dmann_batch_last_error(self._h) or b"").decode())
+
+    def set_tuning(self, chunk: int, blocks_per_cu: int):
+        self._chk(self._L.dmann_batch_set_tuning(self._h, int(chunk), int(blocks_per_cu)))
+
+    def render(self, parts, use_graph: bool = True):
+        """parts: (n, 12) float32 CUDA tensor, kept alive by the caller
+        while a captured graph may replay it."""
+        self._chk(self._L.dmann_batch_render_device(self._h, C.c_void_p(parts.data_ptr()), int(parts.shape[0]),
+                                                    int(bool(use_graph))))
+
+    def synchronize(self):
+        self._chk(self._L.dmann_batch_synchronize(self._h))
+
+    def get_map(self, i: int) -> np.ndarray:
+        out = np.empty(self.npix[i], dtype=np.float64)
+        self._chk(self._L.dmann_batch_get_map(self._h, int(i), C.c_void_p(out.ctypes.data)))
+        return out
+
+    def stats(self, i: int) -> dict:
+        st = DmannStats()
+        self._chk(self._L.dmann_batch_get_stats(self._h, int(i), C.byref(st)))
+        return st.as_dict()
+
+    def replays(self) -> int:
+        return int(self._L.dmann_batch_graph_replays(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.dmann_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ FITS

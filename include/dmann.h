@@ -115,6 +115,12 @@ int dmann_render(dmann_ctx *ctx, const dmann_particle *host_parts, int64_t n);
  * call (dmann_get_map, dmann_synchronize). */
 int dmann_render_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t n);
 
+/* Launch shape and size-class threshold (no counterpart in the reference):
+ * persistent workgroups per CU for the wavefront (medium) and workgroup
+ * (large) footprint kernels, and the estimated footprint size above which a
+ * particle goes to the workgroup kernel.  Defaults 8, 4, 65536. */
+int dmann_set_tuning(dmann_ctx *ctx, int medium_blocks_per_cu, int large_blocks_per_cu, int large_pix);
+
 /* Uses `stream` (a hipStream_t, NULL = the null stream) for later calls. */
 int dmann_set_stream(dmann_ctx *ctx, void *stream);
 
@@ -160,6 +166,29 @@ int64_t dmann_npix(const dmann_ctx *ctx);
 
 /* Frees everything.  Replaces cudaCleaingUp (src/chealpixrender.cu:122-125). */
 void dmann_destroy(dmann_ctx *ctx);
+
+/* ---- batched maps (no counterpart in the reference, which renders one
+ * observer per run; for callers that need many observers / resolutions of
+ * one particle set) ----
+ * All maps of a batch share the particles and the precision; each has its
+ * own observer, rotation and nside (params[i]).  A render computes every map
+ * afresh (clears, one persistent launch over (map, particle-chunk) work
+ * items, FP64 finalisation), and with use_graph != 0 it is captured once in
+ * a hipGraph per (particle pointer, n) and replayed. */
+typedef struct dmann_batch dmann_batch;
+int dmann_batch_create(dmann_batch **b, const dmann_params *params, int nmaps, int device);
+/* particles per work item (1..2^20) and persistent workgroups per CU (1..64) */
+int dmann_batch_set_tuning(dmann_batch *b, int chunk, int blocks_per_cu);
+int dmann_batch_render_device(dmann_batch *b, const dmann_particle *dev_parts, int64_t n, int use_graph);
+int dmann_batch_synchronize(dmann_batch *b);
+int64_t dmann_batch_npix(const dmann_batch *b, int i);
+/* device pointer of finished map i (npix doubles, / dOmega), valid until destroy */
+int dmann_batch_map_device(dmann_batch *b, int i, double **dev_map);
+int dmann_batch_get_map(dmann_batch *b, int i, double *host_map);
+int dmann_batch_get_stats(dmann_batch *b, int i, dmann_stats *st);
+long long dmann_batch_graph_replays(const dmann_batch *b);
+void dmann_batch_destroy(dmann_batch *b);
+const char *dmann_batch_last_error(const dmann_batch *b);
 
 /* HEALPix FITS map file, as MapGenerator::saveToFits writes it
  * (src/mapgenerator.cpp:30-62): RING order, one float64 column.
