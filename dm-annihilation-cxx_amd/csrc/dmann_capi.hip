@@ -33,6 +33,7 @@ struct dmann_ctx {
     MapRef map{nullptr, nullptr, nullptr};
     double *d_out = nullptr; // finalised FP64 map for host reads
     LaunchCfg cfg{};
+    GroupCfg gc{};
     int cus = 0;
     bool timing = false;
     std::vector<hipEvent_t> events; // 4 per timed render call
@@ -63,8 +64,38 @@ static int fail(dmann_ctx *c, int code, const char *fmt, ...)
         if (e_ != hipSuccess) return fail((ctx), DMANN_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+static void free_group(dmann_ctx *c)
+{
+    (void)hipFree(c->gc.keys);
+    (void)hipFree(c->gc.idx);
+    (void)hipFree(c->gc.hist);
+    (void)hipFree(c->gc.sums);
+    c->gc.keys = c->gc.idx = c->gc.hist = c->gc.sums = nullptr;
+}
+
+// Sort-cell buffers for grouped rendering (k > 0); k == 0 frees them.
+static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
+{
+    free_group(c);
+    c->gc.k = k;
+    c->gc.nside_c = nside_c;
+    if (k <= 0 || c->acc == kAccF64) {
+        c->gc.k = 0;
+        return hipSuccess;
+    }
+    const int64_t cap = c->max_parts ? c->max_parts : 1;
+    const int64_t nbins = 12LL * nside_c * nside_c;
+    hipError_t e;
+    if ((e = hipMalloc(&c->gc.keys, cap * sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipMalloc(&c->gc.idx, cap * sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipMalloc(&c->gc.hist, nbins * sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipMalloc(&c->gc.sums, ((nbins + 1023) / 1024) * sizeof(unsigned))) != hipSuccess) return e;
+    return hipSuccess;
+}
+
 static void free_all(dmann_ctx *c)
 {
+    free_group(c);
     hipFree(c->d_parts);
     hipFree(c->d_recs);
     hipFree(c->d_counts);
@@ -147,6 +178,9 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     if ((e = hipMalloc(&c->d_counts, 4 * sizeof(unsigned))) != hipSuccess) return bail("hipMalloc counts", e);
     if ((e = hipMalloc(&c->d_stats, kNumStats * sizeof(unsigned long long))) != hipSuccess)
         return bail("hipMalloc stats", e);
+    c->gc.sort_blocks = dev_cus * 16;
+    c->gc.group_blocks = dev_cus * 8;
+    if ((e = alloc_group(c, 16, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);
     *out = c;
@@ -164,6 +198,17 @@ int dmann_set_tuning(dmann_ctx *c, int medium_blocks_per_cu, int large_blocks_pe
     c->cfg.medium_blocks = cus * medium_blocks_per_cu;
     c->cfg.large_blocks = cus * large_blocks_per_cu;
     c->P.large_pix = large_pix;
+    return DMANN_OK;
+}
+
+int dmann_set_grouping(dmann_ctx *c, int k, int nside_c)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (k < 0 || k > 64) return fail(c, DMANN_ERR_ARG, "group size %d outside 0..64", k);
+    if (nside_c < 1 || nside_c > c->P.nside) return fail(c, DMANN_ERR_ARG, "sort cell nside %d outside 1..nside", nside_c);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, alloc_group(c, k, nside_c));
     return DMANN_OK;
 }
 
@@ -194,7 +239,7 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
         ev = &c->events[c->events.size() - 4];
     }
     HIPCHK(c, launch_render(c->acc, reinterpret_cast<const Particle *>(dev_parts), n, c->P, c->d_recs,
-                            c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev));
+                            c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev, &c->gc));
     return DMANN_OK;
 }
 
@@ -246,7 +291,7 @@ int dmann_render(dmann_ctx *c, const dmann_particle *host_parts, int64_t n)
     if (!c->d_parts) HIPCHK(c, hipMalloc(&c->d_parts, c->max_parts * sizeof(Particle)));
     HIPCHK(c, hipMemcpyAsync(c->d_parts, host_parts, n * sizeof(Particle), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_render(c->acc, c->d_parts, n, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg,
-                            c->stream));
+                            c->stream, nullptr, &c->gc));
     // the staging buffer is reused by the next call: wait for this one
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return DMANN_OK;
@@ -334,6 +379,27 @@ static void fill_devparams(const dmann_params *p, DevParams &P)
     P.dOmega = 4 * kPi / (double)P.npix;
     P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592));
     P.large_pix = 1 << 16;
+}
+
+int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)
+{
+    if (!p || !q) return -1;
+    DevParams P;
+    fill_devparams(p, P);
+    DiscRec rec;
+    bool bad = false;
+    const int cls = prepass_one(P, *reinterpret_cast<const Particle *>(q), rec, bad);
+    if (bad || cls != kClsMedium) return -2;
+    const PGeo g = particle_geo(rec, P.nside);
+    int64_t mism = 0;
+    for (int gi = 0; gi < g.npart; ++gi) {
+        const RingRec a = ring_setup(g, P.nside, gi + g.nnorth);
+        const RingLite b = ring_lite(g, P.nside, g.irmin + gi);
+        if (a.cnt != b.cnt || (a.cnt && a.col0 != b.col0) || a.A != b.A || a.B != b.B || a.offm != b.offm ||
+            a.step != b.step || (a.cnt && a.kc_rel != b.kc_rel) || a.startpix != b.startpix || a.nr != b.nr)
+            mism++;
+    }
+    return mism;
 }
 
 int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int32_t *pix, float *w, int64_t cap,

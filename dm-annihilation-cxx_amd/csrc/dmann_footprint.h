@@ -195,6 +195,64 @@ DMANN_HD RingRec ring_setup(const PGeo &g, int nside, int gi)
     return o;
 }
 
+// Partial ring `iz` of a disc that reaches no pole (the medium class), with
+// the same column range as ring_setup but cheaper: the half-width dphi is
+// first computed with FP32 atan2 from the FP64 terms x and 1 - z^2 - x^2;
+// the floor() decisions for ip_lo / ip_hi are kept only when the boundary
+// lies farther than a bound on the FP32 error (relative 1e-6 of dphi) from
+// an integer, otherwise dphi is recomputed in FP64 exactly as ring_setup
+// does.  Only the FP32 weight terms are filled (not z, s, shift).
+struct RingLite {
+    int cnt, col0, kc_rel, startpix, nr;
+    float A, B, offm, step;
+};
+
+DMANN_HD RingLite ring_lite(const PGeo &g, int nside, int iz)
+{
+    RingLite o;
+    int shifted;
+    ring_info(nside, iz, o.startpix, o.nr, shifted);
+    const double shift = shifted ? 0.5 : 0.0;
+    const double z = ring2z(nside, iz);
+    const double fnr = (double)o.nr;
+    const int kc = (int)floor(fnr * kInvTwoPi * g.phi - shift + 0.5);
+    o.cnt = 0;
+    o.col0 = kc;
+    const double x = (g.cosr - z * g.z0) * g.xa;
+    const double ysq = 1 - z * z - x * x;
+    if (ysq > 0) {
+        const float df = atan2f(sqrtf((float)ysq), (float)x);
+        double dphi = (double)df;
+        double ulo = fnr * kInvTwoPi * (g.phi - dphi) - shift;
+        double uhi = fnr * kInvTwoPi * (g.phi + dphi) - shift;
+        const double margin = fnr * kInvTwoPi * dphi * 1e-6 + 1e-9;
+        const double flo = ulo - floor(ulo), fhi = uhi - floor(uhi);
+        if (flo < margin || flo > 1.0 - margin || fhi < margin || fhi > 1.0 - margin) {
+            dphi = atan2(sqrt(ysq), x);
+            ulo = fnr * kInvTwoPi * (g.phi - dphi) - shift;
+            uhi = fnr * kInvTwoPi * (g.phi + dphi) - shift;
+        }
+        if (dphi > 0) {
+            const int ip_lo = (int)floor(ulo) + 1;
+            const int ip_hi = (int)floor(uhi);
+            if (ip_lo <= ip_hi) {
+                o.col0 = ip_lo;
+                o.cnt = ip_hi - ip_lo + 1;
+            }
+        }
+    }
+    const double s = sqrt((1 - z) * (1 + z));
+    double A = (1.0 - (z * g.z0 + s * g.s0)) * 0.5;
+    if (A < 0) A = 0;
+    const double step = kTwoPi / fnr;
+    o.A = (float)A;
+    o.B = (float)(s * g.s0);
+    o.offm = (float)(((double)kc + shift) * step - g.phi);
+    o.step = (float)step;
+    o.kc_rel = kc - o.col0;
+    return o;
+}
+
 // Address parameters of a ring whose first footprint pixel has flat index
 // pre_start: pixel(k) = k + abase - (k >= kwrap ? nr : 0).  Columns run
 // col0 .. col0+cnt-1 unwrapped, with col0 in [-nr/2-1, nr) and cnt <= nr.

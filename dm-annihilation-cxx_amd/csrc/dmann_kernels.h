@@ -65,11 +65,27 @@ struct LaunchCfg {
     int large_blocks;
 };
 
+// Grouped rendering of the medium class (dmann_group.hip): counting sort by
+// coarse sky cell, then K neighbours per wavefront with LDS accumulation.
+struct GroupCfg {
+    int k;            // particles per group (0: per-particle kernel instead)
+    int nside_c;      // sort cell resolution
+    int sort_blocks;
+    int group_blocks; // single-wave workgroups
+    unsigned *keys;   // max_parts
+    unsigned *idx;    // max_parts
+    unsigned *hist;   // 12 nside_c^2
+    unsigned *sums;   // ceil(12 nside_c^2 / 1024)
+};
+
+hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, const GroupCfg &gc, MapRef map,
+                        unsigned long long *stats, hipStream_t st);
+
 // ev (optional): 4 events recorded before the pre-pass, after it, after the
 // sky-covering kernel and after the medium kernel.
 hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                          unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
-                         hipStream_t st, hipEvent_t *ev = nullptr);
+                         hipStream_t st, hipEvent_t *ev = nullptr, const GroupCfg *gc = nullptr);
 hipError_t launch_adjoint(const Particle *d_parts, int64_t n, const DevParams &P, const double *gmap, double *out,
                           int blocks, hipStream_t st);
 hipError_t launch_batch(int acc, const Particle *d_parts, int64_t n, const DevParams *Ps, const MapRef *maps,

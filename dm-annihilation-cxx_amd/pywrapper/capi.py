@@ -49,8 +49,8 @@ class DmannStats(C.Structure):
 EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_render_device",
            "dmann_set_stream", "dmann_synchronize", "dmann_get_map", "dmann_get_map_device",
            "dmann_reset", "dmann_get_stats", "dmann_npix", "dmann_destroy",
-           "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint",
-           "dmann_timing", "dmann_adjoint_device", "dmann_last_error", "dmann_set_tuning",
+           "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint", "dmann_host_ring_check",
+           "dmann_timing", "dmann_adjoint_device", "dmann_last_error", "dmann_set_tuning", "dmann_set_grouping",
            "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device",
            "dmann_batch_synchronize", "dmann_batch_npix", "dmann_batch_map_device", "dmann_batch_get_map",
            "dmann_batch_get_stats", "dmann_batch_graph_replays", "dmann_batch_destroy",
@@ -94,8 +94,11 @@ def load_library():
     L.dmann_host_footprint.argtypes = [C.POINTER(DmannParams), vp, vp, vp, i64, C.POINTER(C.c_double),
                                        C.POINTER(C.c_int)]
     L.dmann_host_footprint.restype = i64
+    L.dmann_host_ring_check.argtypes = [C.POINTER(DmannParams), vp]
+    L.dmann_host_ring_check.restype = i64
     L.dmann_adjoint_device.argtypes = [vp, vp, i64, vp, vp]
     L.dmann_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.dmann_set_grouping.argtypes = [vp, C.c_int, C.c_int]
     L.dmann_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.dmann_last_error.argtypes = [vp]
     L.dmann_last_error.restype = C.c_char_p
@@ -255,6 +258,11 @@ class Projector:
     def set_tuning(self, medium_blocks_per_cu: int = 8, large_blocks_per_cu: int = 4, large_pix: int = 65536):
         _check(self._L.dmann_set_tuning(self._h, int(medium_blocks_per_cu), int(large_blocks_per_cu),
                                         int(large_pix)), self._h)
+
+    def set_grouping(self, k: int = 16, nside_c: int | None = None):
+        """K neighbours per wavefront (0: one particle per wavefront)."""
+        ns = self.params.nside
+        _check(self._L.dmann_set_grouping(self._h, int(k), int(nside_c or max(1, ns // 8))), self._h)
 
     def timing(self, enable: bool):
         """Returns ({'prepass','large','medium'}: summed ms, calls) of the

@@ -121,6 +121,13 @@ int dmann_render_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t
  * particle goes to the workgroup kernel.  Defaults 8, 4, 65536. */
 int dmann_set_tuning(dmann_ctx *ctx, int medium_blocks_per_cu, int large_blocks_per_cu, int large_pix);
 
+/* Grouped rendering of the medium class (no counterpart in the reference):
+ * particles are sorted by RING cell at nside_c (1..nside) and K (1..64)
+ * neighbours share one wavefront and an LDS window, cutting global atomics
+ * ~K-fold; k = 0 renders one particle per wavefront with per-pixel atomics.
+ * Defaults K = 16, nside_c = nside / 8.  Ignored by the FP64 form. */
+int dmann_set_grouping(dmann_ctx *ctx, int k, int nside_c);
+
 /* Uses `stream` (a hipStream_t, NULL = the null stream) for later calls. */
 int dmann_set_stream(dmann_ctx *ctx, void *stream);
 
@@ -206,6 +213,13 @@ int dmann_read_healpix_fits(const char *path, double *map, int64_t cap, int *nsi
  * direction.  A debugging and test aid with no counterpart in the reference. */
 int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int32_t *pix, float *w,
                              int64_t cap, double *flux, int *kind);
+
+/* Host-side check of the medium-class ring walk: the number of rings of a
+ * medium-class particle where the fast ring setup (FP32 half-width with an
+ * FP64 recomputation near column boundaries) differs from the exact FP64
+ * setup (0 expected); -2 when the particle is not in the medium class.  A
+ * test aid with no counterpart in the reference. */
+int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q);
 
 /* Message of the last failed call on ctx (or of the last failed
  * dmann_create when ctx is NULL). */

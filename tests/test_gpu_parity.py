@@ -98,3 +98,24 @@ def test_chunked_accumulation_matches_single_call(pw, gpu, nfw_1e5):
     assert st1 == st4
     rep = rel_err_report(m4, m1)
     assert rep["max_rel"] <= 1e-12
+
+
+@pytest.mark.parametrize("k", [0, 1, 16, 64])
+@pytest.mark.parametrize("precision", ["compensated", "fp32"])
+def test_grouping_sizes(pw, orc, gpu, precision, k):
+    """Grouped rendering (K neighbours per wavefront sharing an LDS window;
+    K = 0 is the per-particle kernel) against the oracle at nside 512."""
+    print(f"case grouping k={k} precision={precision}", flush=True)
+    recs = pw.synth.to_records(pw.synth.make("nfw_sub100_1e7", n=20000))
+    ref, ost = _run_oracle(orc, pw, recs, 512)
+    p = pw.params_for_observer(512, precision)
+    with pw.Projector(p, len(recs)) as pr:
+        pr.set_grouping(k)
+        pr.render(recs)
+        m = pr.get_map()
+        st = pr.stats()
+    _check_counts(st, ost)
+    rep = rel_err_report(m, ref)
+    print(rep)
+    tol, ttol = TOL[precision]
+    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
