@@ -44,7 +44,7 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="nfw_sub2000_near_1p1e9")
     ap.add_argument("--n", type=int, default=None, help="override particle count (model sample)")

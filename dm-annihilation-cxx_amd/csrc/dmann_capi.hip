@@ -70,7 +70,8 @@ static void free_group(dmann_ctx *c)
     (void)hipFree(c->gc.idx);
     (void)hipFree(c->gc.hist);
     (void)hipFree(c->gc.sums);
-    c->gc.keys = c->gc.idx = c->gc.hist = c->gc.sums = nullptr;
+    (void)hipFree(c->gc.work);
+    c->gc.keys = c->gc.idx = c->gc.hist = c->gc.sums = c->gc.work = nullptr;
 }
 
 // Sort-cell buffers for grouped rendering (k > 0); k == 0 frees them.
@@ -90,6 +91,7 @@ static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
     if ((e = hipMalloc(&c->gc.idx, cap * sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMalloc(&c->gc.hist, nbins * sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMalloc(&c->gc.sums, ((nbins + 1023) / 1024) * sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipMalloc(&c->gc.work, sizeof(unsigned))) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -145,7 +147,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     P.npix = (int)c->npix;
     P.dOmega = 4 * kPi / (double)c->npix;              // Map::setNside, datatypes.h:248-253
     P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592)); // src/genmap.cpp:38
-    P.large_pix = 1 << 16;
+    P.large_pix = 1 << 22;
 
     hipError_t e = hipSetDevice(device);
     int dev_cus = 256;
@@ -378,7 +380,7 @@ static void fill_devparams(const dmann_params *p, DevParams &P)
     P.npix = (int)npix_of(p->nside);
     P.dOmega = 4 * kPi / (double)P.npix;
     P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592));
-    P.large_pix = 1 << 16;
+    P.large_pix = 1 << 22;
 }
 
 int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)

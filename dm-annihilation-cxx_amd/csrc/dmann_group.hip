@@ -14,8 +14,11 @@
 //      lane holds a distinct pixel, and members are processed in turn);
 //   3. the band is flushed with one global atomic (form) per non-zero pixel.
 // A group's pixels are written ~K times less often than with per-pixel
-// atomics.  The compensated form keeps (hi, lo) pairs in LDS (TwoSum per
-// deposit) and flushes them with the two-float atomic pair.
+// atomics.  The window holds FP32 partial sums of at most K deposits per
+// pixel (one per member; error <= (K-1) u of the partial sum); the
+// compensated form flushes them with the two-float atomic pair (exact
+// TwoSum), so the sum over groups -- 1e4-1e5 terms per pixel -- stays
+// compensated.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -27,7 +30,7 @@
 namespace dmann {
 
 // LDS budget of one wave: 20 KB, so 8 single-wave blocks fit a 160 KB CU.
-constexpr int kGroupCapF32 = 3936; // FP32 slots (compensated: half as many pairs)
+constexpr int kGroupCapF32 = 3936; // FP32 window slots
 constexpr int kSeg = 8;            // pixels per segment
 
 struct GroupLDS {
@@ -121,7 +124,6 @@ template <int MODE, int ACC>
 __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, float K4, float sc, bool direct,
                                                 const MapRef &map)
 {
-    float2 *acc2 = reinterpret_cast<float2 *>(S.acc);
     const int s0 = (lane * NS) >> 6, s1 = ((lane + 1) * NS) >> 6;
     const int nmax = (NS + 63) >> 6;
     double total = 0.0;
@@ -151,17 +153,7 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
                     if (t < len) {
                         const float v = weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, 0.0f, 0.0f, K4) * sc;
                         if (!direct && off < w.y) {
-                            const int slot = w.x + off;
-                            if constexpr (ACC == kAccComp) {
-                                float2 c2 = acc2[slot];
-                                const float s2 = c2.x + v;
-                                const float bb = s2 - c2.x;
-                                c2.y += (c2.x - (s2 - bb)) + (v - bb);
-                                c2.x = s2;
-                                acc2[slot] = c2;
-                            } else {
-                                S.acc[slot] += v;
-                            }
+                            S.acc[w.x + off] += v;
                         } else {
                             const int2 d = S.rd[q];
                             int col = d.x + t0 + t;
@@ -186,22 +178,26 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
 template <int ACC>
 __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ recs, const unsigned *__restrict__ idx,
                                                    const unsigned *__restrict__ counts, int nside, int K, MapRef map,
-                                                   unsigned long long *__restrict__ stats)
+                                                   unsigned long long *__restrict__ stats, unsigned *__restrict__ work)
 {
     __shared__ GroupLDS S;
-    constexpr int CAP = (ACC == kAccComp) ? kGroupCapF32 / 2 : kGroupCapF32;
-    float2 *acc2 = reinterpret_cast<float2 *>(S.acc);
+    constexpr int CAP = kGroupCapF32;
     const int lane = threadIdx.x;
     const unsigned count = counts[0];
     const unsigned ngroups = (count + (unsigned)K - 1u) / (unsigned)K;
     unsigned long long n_ext = 0, n_fb = 0, n_contrib = 0;
-    for (unsigned grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    // groups are taken from a counter: footprint sizes vary a lot
+    for (;;) {
+        unsigned grp = 0;
+        if (lane == 0) grp = atomicAdd(work, 1u);
+        grp = __shfl(grp, 0, 64);
+        if (grp >= ngroups) break;
         const unsigned m0 = grp * (unsigned)K;
         const int nm = (int)((m0 + (unsigned)K < count ? m0 + (unsigned)K : count) - m0);
         // ---- phase 1: normalisation of every member
-        float my_scale = 0.0f;
+        float my_scale = 0.0f, my_K4 = 0.0f;
         int my_lo = 0x7fffffff, my_hi = -0x7fffffff;
-        double my_phi = 0.0, my_dphi = 0.0;
+        double my_phi = 0.0, my_dphi = 0.0, my_z0 = 0.0, my_s0 = 0.0, my_xa = 0.0, my_cosr = 0.0;
         bool my_on = false;
         for (int j = 0; j < nm; ++j) {
             const PGeo g = particle_geo(recs[idx[m0 + j]], nside);
@@ -228,6 +224,11 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
                     my_hi = g.irmax;
                     my_phi = g.phi;
                     my_dphi = asin(fmin(1.0, sin(g.rad) / g.s0));
+                    my_z0 = g.z0;
+                    my_s0 = g.s0;
+                    my_xa = g.xa;
+                    my_cosr = g.cosr;
+                    my_K4 = g.K4;
                     my_on = true;
                 }
             }
@@ -279,11 +280,7 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
                 }
                 if (lane == 0) S.boff[0] = 0;
                 Sb = __shfl(incl, BH - 1, 64);
-                if constexpr (ACC == kAccComp) {
-                    for (int t = lane; t < Sb; t += 64) acc2[t] = make_float2(0.0f, 0.0f);
-                } else {
-                    for (int t = lane; t < Sb; t += 64) S.acc[t] = 0.0f;
-                }
+                for (int t = lane; t < Sb; t += 64) S.acc[t] = 0.0f;
             } else {
                 // direct deposits still need the band table for slot terms
                 if (lane < BH) {
@@ -298,7 +295,14 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
                 const int lo = __shfl(my_lo, j, 64), hi = __shfl(my_hi, j, 64);
                 if (!__shfl((int)my_on, j, 64) || hi < b0 || lo > b1) continue;
                 const float sc = __shfl(my_scale, j, 64);
-                const PGeo g = particle_geo(recs[idx[m0 + j]], nside);
+                // the member's ring-walk terms, from its lane (computed in phase 1)
+                PGeo g;
+                g.phi = __shfl(my_phi, j, 64);
+                g.z0 = __shfl(my_z0, j, 64);
+                g.s0 = __shfl(my_s0, j, 64);
+                g.xa = __shfl(my_xa, j, 64);
+                g.cosr = __shfl(my_cosr, j, 64);
+                g.K4 = __shfl(my_K4, j, 64);
                 const int rlo = b0 > lo ? b0 : lo, rhi = b1 < hi ? b1 : hi;
                 const int NS = ring_table(g, nside, rlo, rhi - rlo + 1, S, lane, b0);
                 walk_segments<1, ACC>(S, NS, lane, g.K4, sc, direct, map);
@@ -313,19 +317,8 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
                     int col = S.bcl[jb] + (t - S.boff[jb]);
                     if (col >= nrb) col -= nrb;
                     const int pix = S.bstart[jb] + col;
-                    if constexpr (ACC == kAccComp) {
-                        const float2 c2 = acc2[t];
-                        if (c2.x != 0.0f || c2.y != 0.0f) {
-                            const float old = unsafeAtomicAdd(map.f32 + pix, c2.x);
-                            const float s2 = old + c2.x;
-                            const float bb = s2 - old;
-                            const float err = (old - (s2 - bb)) + (c2.x - bb) + c2.y;
-                            if (err != 0.0f) unsafeAtomicAdd(map.lo + pix, err);
-                        }
-                    } else {
-                        const float v = S.acc[t];
-                        if (v != 0.0f) acc_add<ACC>(map, pix, v, 0.0);
-                    }
+                    const float v = S.acc[t];
+                    if (v != 0.0f) acc_add<ACC>(map, pix, v, 0.0);
                 }
                 wave_sync();
             }
@@ -429,6 +422,7 @@ __global__ void k_zero_u32(unsigned *p, int n)
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, const GroupCfg &gc, MapRef map,
                         unsigned long long *stats, hipStream_t st)
 {
+    k_zero_u32<<<1, 64, 0, st>>>(gc.work, 1);
     const int nbins = 12 * gc.nside_c * gc.nside_c;
     const int nb = (nbins + 1023) / 1024;
     k_zero_u32<<<nb < 4096 ? nb : 4096, 256, 0, st>>>(gc.hist, nbins);
@@ -438,9 +432,9 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
     k_scan_add<<<nb, 256, 0, st>>>(gc.hist, nbins, gc.sums);
     k_sort_scatter<<<gc.sort_blocks, 256, 0, st>>>(gc.keys, counts, gc.hist, gc.idx);
     if (acc == kAccComp)
-        k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats);
+        k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work);
     else
-        k_disc_group<kAccF32><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats);
+        k_disc_group<kAccF32><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work);
     return hipGetLastError();
 }
 

@@ -76,6 +76,7 @@ struct GroupCfg {
     unsigned *idx;    // max_parts
     unsigned *hist;   // 12 nside_c^2
     unsigned *sums;   // ceil(12 nside_c^2 / 1024)
+    unsigned *work;   // group counter
 };
 
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, const GroupCfg &gc, MapRef map,

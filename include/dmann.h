@@ -118,7 +118,7 @@ int dmann_render_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t
 /* Launch shape and size-class threshold (no counterpart in the reference):
  * persistent workgroups per CU for the wavefront (medium) and workgroup
  * (large) footprint kernels, and the estimated footprint size above which a
- * particle goes to the workgroup kernel.  Defaults 8, 4, 65536. */
+ * particle goes to the workgroup kernel.  Defaults 8, 4, 4194304. */
 int dmann_set_tuning(dmann_ctx *ctx, int medium_blocks_per_cu, int large_blocks_per_cu, int large_pix);
 
 /* Grouped rendering of the medium class (no counterpart in the reference):
