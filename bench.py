@@ -53,6 +53,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 GPU map that the per-pixel errors are measured against")
+    ap.add_argument("--host-n", type=int, default=0,
+                    help="also time N particles handed over in pinned host memory (PCIe-inclusive rate)")
     return ap.parse_args()
 
 
@@ -86,6 +90,48 @@ def time_form(pw, torch, parts, nside, precision, steps, warmup):
     sec = (t1 - t0) / steps
     return {"sec": sec, "kernel_ms": {k: v / max(1, calls) for k, v in kms.items()}, "stats": stats,
             "map": out}
+
+
+def pixel_errors(torch, m, ref):
+    """Per-pixel relative error on pixels above 1e-6 of the map maximum and
+    the relative error of the total, against the FP64 map."""
+    big = ref > 1e-6 * ref.max()
+    rel = ((m - ref).abs() / ref.abs().clamp_min(1e-300))[big]
+    return {"max_rel": float(rel.max()), "pixels_checked": int(big.sum()),
+            "total_rel": float(abs(m.sum() - ref.sum()) / ref.sum())}
+
+
+def host_input_rate(pw, torch, parts, nside, precision, n):
+    """Mparticles/s when the particles arrive in pinned host memory, the
+    host-buffer call of the C ABI: H2D copy of the 48-byte records + render."""
+    import numpy as np
+    sample = parts[:n].cpu().pin_memory()
+    host = sample.numpy()
+    p = pw.params_for_observer(nside, precision)
+    out = torch.empty(12 * nside * nside, dtype=torch.float64, device="cuda")
+    pr = pw.Projector(p, n)
+    try:
+        pr.render(host[:min(n, 1 << 20)])
+        pr.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pr.render(host)
+        pr.get_map_device(out)
+        torch.cuda.synchronize()
+        t_host = time.perf_counter() - t0
+        dev = sample.cuda()
+        pr.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pr.render(dev)
+        pr.get_map_device(out)
+        torch.cuda.synchronize()
+        t_dev = time.perf_counter() - t0
+    finally:
+        pr.close()
+    del np
+    return {"n": n, "value": n / t_host / 1e6, "device_resident_value": n / t_dev / 1e6,
+            "h2d_GBps": 48 * n / max(t_host - t_dev, 1e-9) / 1e9, "unit": "Mparticles/s"}
 
 
 def total_flux_check(torch, parts, params, m, nside):
@@ -129,10 +175,19 @@ def cpu_baseline(pw, args, nside):
     t0 = time.perf_counter()
     _, st = oracle.genmap(recs, cfg, nthreads=cores, finalize=False)
     dt = time.perf_counter() - t0
+    # the reference's generator is single-threaded: time one core on a
+    # quarter of the sample as well
+    m1 = max(500, m2 // (4 * cores))
+    recs1 = pw.synth.to_records(hm.sample(m1))
+    t0 = time.perf_counter()
+    _, st1 = oracle.genmap(recs1, cfg, nthreads=1, finalize=False)
+    dt1 = time.perf_counter() - t0
     return {"value": m2 / dt / 1e6, "unit": "Mparticles/s", "cores": cores, "kind": "port",
             "sample": f"{m2} particles of the same model (16 index windows + near-observer share), "
                       f"nside {nside}, {st['contributions']} footprint pixels, {dt:.1f} s with "
-                      f"{cores} OpenMP threads"}
+                      f"{cores} OpenMP threads",
+            "single_core": {"value": m1 / dt1 / 1e6, "sample": f"{m1} particles, {st1['contributions']} "
+                            f"footprint pixels, {dt1:.1f} s on 1 thread"}}
 
 
 def main():
@@ -161,23 +216,44 @@ def main():
     params = pw.params_for_observer(nside, args.precision)
     flux_err = total_flux_check(torch, parts, params, res["map"], nside)
     plain = None
+    pl_map = None
     if not args.no_plain and args.precision != "fp32":
         pl = time_form(pw, torch, parts, nside, "fp32", args.steps, args.warmup)
         pl_err = total_flux_check(torch, parts, params, pl["map"], nside)
         log(f"fp32: {pl['sec'] * 1e3:.1f} ms/step, kernels {pl['kernel_ms']}")
         plain = {"value": n / pl["sec"] / 1e6, "ms_per_step": pl["sec"] * 1e3,
-                 "medium_kernel_ms": pl["kernel_ms"]["medium"], "total_flux_rel_err": pl_err,
+                 "kernel_ms": pl["kernel_ms"], "total_flux_rel_err": pl_err,
                  "compensated_over_plain_time": res["sec"] / pl["sec"]}
+        pl_map = pl["map"]
         del pl
+    vs_fp64 = None
+    if not args.no_fp64 and args.precision != "fp64":
+        # the FP64 form (FP64 acos/exp weights, FP64 atomics) is the GPU oracle;
+        # one untimed-warmup-free render is enough for the error figures
+        ref = time_form(pw, torch, parts, nside, "fp64", 1, 0)
+        log(f"fp64: {ref['sec'] * 1e3:.1f} ms/step")
+        vs_fp64 = {args.precision: pixel_errors(torch, res["map"], ref["map"]),
+                   "fp64_ms": ref["sec"] * 1e3,
+                   "fp64_total_flux_rel_err": total_flux_check(torch, parts, params, ref["map"], nside)}
+        if pl_map is not None:
+            vs_fp64["fp32"] = pixel_errors(torch, pl_map, ref["map"])
+        log(f"vs fp64: {vs_fp64}")
+        del ref
+    del pl_map
+    host_in = None
+    if args.host_n > 0:
+        host_in = host_input_rate(pw, torch, parts, nside, args.precision, min(args.host_n, n))
+        log(f"host input: {host_in}")
     C = res["stats"]["contributions"]
+    C_large = res["stats"]["large_contributions"]
+    C_med = C - C_large
     kms = res["kernel_ms"]
+    # dominant kernel: the medium footprint kernel (grouped), timed alone
     t_med = kms["medium"] / 1e3
-    # dominant kernel: the medium footprint kernel
-    frac_med = 1.0 - res["stats"]["large"] / max(1, res["stats"]["extended"])
-    valu_ops = EVALS_PER_CONTRIB * SLOTS_PER_EVAL * C
+    achieved_valu = EVALS_PER_CONTRIB * SLOTS_PER_EVAL * C_med / t_med / 1e12
+    both = EVALS_PER_CONTRIB * SLOTS_PER_EVAL * C / ((kms["medium"] + kms["large"]) / 1e3) / 1e12
     s_bytes = 8 if args.precision != "fp32" else 4
     hbm_bytes = 48 * n + 2 * s_bytes * 12 * nside * nside
-    achieved_valu = valu_ops / (t_med + kms["large"] / 1e3) / 1e12
     roofline = {
         "bound": "valu",
         "achieved": achieved_valu,
@@ -185,13 +261,13 @@ def main():
         "unit": "Tlane-op/s",
         "frac": achieved_valu / VALU_PEAK_TOPS,
         "traffic": None,
-        "kernel": "k_disc_medium + k_disc_large (footprint kernels)",
+        "kernel": "k_disc_group (medium footprint kernel)",
         "kernel_ms": kms,
-        "contributions": C,
+        "contributions": {"medium": C_med, "large": C_large},
+        "footprint_kernels_frac": both / VALU_PEAK_TOPS,
         "hbm": {"achieved": hbm_bytes / res["sec"] / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": hbm_bytes / res["sec"] / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": hbm_bytes},
     }
-    del frac_med
     line = {
         "metric": METRIC,
         "value": n / res["sec"] / 1e6,
@@ -212,8 +288,11 @@ def main():
                    "contributions": C},
         "total_flux_rel_err": flux_err,
         "plain_fp32": plain,
+        "vs_fp64": vs_fp64,
         "roofline": roofline,
     }
+    if host_in is not None:
+        line["host_input"] = host_in
     if not args.no_cpu:
         del parts
         torch.cuda.empty_cache()

@@ -261,6 +261,7 @@ int dmann_batch_get_stats(dmann_batch *b, int i, dmann_stats *st)
     st->extended = (int64_t)h[kStatExt];
     st->contributions = (int64_t)h[kStatContrib];
     st->large = 0;
+    st->large_contributions = 0;
     return DMANN_OK;
 }
 

@@ -36,7 +36,7 @@ struct dmann_ctx {
     GroupCfg gc{};
     int cus = 0;
     bool timing = false;
-    std::vector<hipEvent_t> events; // 4 per timed render call
+    std::vector<hipEvent_t> events; // 5 per timed render call
     char msg[512] = {0};
 };
 
@@ -233,12 +233,12 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
     HIPCHK(c, hipSetDevice(c->device));
     hipEvent_t *ev = nullptr;
     if (c->timing) {
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < 5; i++) {
             hipEvent_t e;
             HIPCHK(c, hipEventCreate(&e));
             c->events.push_back(e);
         }
-        ev = &c->events[c->events.size() - 4];
+        ev = &c->events[c->events.size() - 5];
     }
     HIPCHK(c, launch_render(c->acc, reinterpret_cast<const Particle *>(dev_parts), n, c->P, c->d_recs,
                             c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev, &c->gc));
@@ -264,16 +264,16 @@ int dmann_timing(dmann_ctx *c, int enable, double *ms, int *calls)
     HIPCHK(c, hipSetDevice(c->device));
     if (ms || calls) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        double acc[3] = {0, 0, 0};
-        for (size_t i = 0; i + 3 < c->events.size(); i += 4)
-            for (int k = 0; k < 3; k++) {
+        double acc[4] = {0, 0, 0, 0};
+        for (size_t i = 0; i + 4 < c->events.size(); i += 5)
+            for (int k = 0; k < 4; k++) {
                 float t = 0;
                 HIPCHK(c, hipEventElapsedTime(&t, c->events[i + k], c->events[i + k + 1]));
                 acc[k] += t;
             }
         if (ms)
-            for (int k = 0; k < 3; k++) ms[k] = acc[k];
-        if (calls) *calls = (int)(c->events.size() / 4);
+            for (int k = 0; k < 4; k++) ms[k] = acc[k];
+        if (calls) *calls = (int)(c->events.size() / 5);
     }
     drop_events(c);
     c->timing = enable != 0;
@@ -353,6 +353,7 @@ int dmann_get_stats(dmann_ctx *c, dmann_stats *st)
     st->extended = (int64_t)h[kStatExt];
     st->contributions = (int64_t)h[kStatContrib];
     st->large = (int64_t)h[kStatLarge];
+    st->large_contributions = (int64_t)h[kStatLargeContrib];
     return DMANN_OK;
 }
 

@@ -420,7 +420,7 @@ __global__ void k_zero_u32(unsigned *p, int n)
 }
 
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, const GroupCfg &gc, MapRef map,
-                        unsigned long long *stats, hipStream_t st)
+                        unsigned long long *stats, hipStream_t st, hipEvent_t sorted)
 {
     k_zero_u32<<<1, 64, 0, st>>>(gc.work, 1);
     const int nbins = 12 * gc.nside_c * gc.nside_c;
@@ -431,6 +431,7 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
     k_scan_sums<<<1, 256, 0, st>>>(gc.sums, nb);
     k_scan_add<<<nb, 256, 0, st>>>(gc.hist, nbins, gc.sums);
     k_sort_scatter<<<gc.sort_blocks, 256, 0, st>>>(gc.keys, counts, gc.hist, gc.idx);
+    if (sorted) (void)hipEventRecord(sorted, st);
     if (acc == kAccComp)
         k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work);
     else

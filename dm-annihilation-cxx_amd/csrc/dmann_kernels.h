@@ -25,6 +25,7 @@ enum {
     kStatContrib = 4,
     kStatLarge = 5,
     kStatBad = 6,
+    kStatLargeContrib = 7,
     kNumStats = 8
 };
 
@@ -80,7 +81,8 @@ struct GroupCfg {
 };
 
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, const GroupCfg &gc, MapRef map,
-                        unsigned long long *stats, hipStream_t st);
+                        unsigned long long *stats, hipStream_t st,
+                        hipEvent_t sorted = nullptr);
 
 // ev (optional): 4 events recorded before the pre-pass, after it, after the
 // sky-covering kernel and after the medium kernel.

@@ -352,6 +352,7 @@ __global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ 
                     atomicAdd(&stats[kStatExt], 1ull);
                     atomicAdd(&stats[kStatLarge], 1ull);
                     atomicAdd(&stats[kStatContrib], (unsigned long long)Ttot);
+                    atomicAdd(&stats[kStatLargeContrib], (unsigned long long)Ttot);
                 }
             }
         }
@@ -633,12 +634,13 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
     k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, map, stats);
     if (ev) (void)hipEventRecord(ev[2], st);
     if (ACC != kAccF64 && gc && gc->k > 0) {
-        const hipError_t e = launch_group(ACC, recs, counts, P.nside, *gc, map, stats, st);
+        const hipError_t e = launch_group(ACC, recs, counts, P.nside, *gc, map, stats, st, ev ? ev[3] : nullptr);
         if (e != hipSuccess) return e;
     } else {
+        if (ev) (void)hipEventRecord(ev[3], st);
         k_disc_medium<ACC><<<cfg.medium_blocks, 256, 0, st>>>(recs, counts, P.nside, map, stats);
     }
-    if (ev) (void)hipEventRecord(ev[3], st);
+    if (ev) (void)hipEventRecord(ev[4], st);
     return hipGetLastError();
 }
 

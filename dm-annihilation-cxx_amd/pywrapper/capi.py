@@ -40,7 +40,8 @@ class DmannParams(C.Structure):
 
 class DmannStats(C.Structure):
     _fields_ = [("kept", C.c_int64), ("subpixel", C.c_int64), ("fallback", C.c_int64),
-                ("extended", C.c_int64), ("contributions", C.c_int64), ("large", C.c_int64)]
+                ("extended", C.c_int64), ("contributions", C.c_int64), ("large", C.c_int64),
+                ("large_contributions", C.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -265,13 +266,15 @@ class Projector:
         _check(self._L.dmann_set_grouping(self._h, int(k), int(nside_c or max(1, ns // 8))), self._h)
 
     def timing(self, enable: bool):
-        """Returns ({'prepass','large','medium'}: summed ms, calls) of the
-        render calls since the last timing() call, then enables/disables
-        event recording for the next ones."""
-        ms = (C.c_double * 3)()
+        """Returns ({'prepass','large','sort','medium'}: summed ms, calls)
+        of the render calls since the last timing() call, then
+        enables/disables event recording for the next ones.  'medium' is
+        the medium footprint kernel alone, 'sort' the coarse-cell sort that
+        precedes it."""
+        ms = (C.c_double * 4)()
         calls = C.c_int()
         _check(self._L.dmann_timing(self._h, int(bool(enable)), ms, C.byref(calls)), self._h)
-        return {"prepass": ms[0], "large": ms[1], "medium": ms[2]}, calls.value
+        return {"prepass": ms[0], "large": ms[1], "sort": ms[2], "medium": ms[3]}, calls.value
 
     def reset(self) -> None:
         _check(self._L.dmann_reset(self._h), self._h)

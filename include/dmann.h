@@ -87,6 +87,7 @@ typedef struct dmann_stats {
     int64_t extended;      /* spread over their footprint                         */
     int64_t contributions; /* total footprint pixels of extended particles        */
     int64_t large;         /* extended particles rendered by the sky-covering path */
+    int64_t large_contributions; /* footprint pixels of those particles           */
 } dmann_stats;
 
 typedef struct dmann_ctx dmann_ctx;
@@ -148,10 +149,11 @@ int dmann_adjoint_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_
 
 /* Per-kernel timing with HIP events on the context's stream.  Reads the
  * summed durations (ms) of the kernels of the render calls made since the
- * last call -- ms[0] pre-pass, ms[1] sky-covering kernel, ms[2] medium
- * kernel -- and their number (both may be NULL), then turns event recording
- * on (enable != 0) or off for the following render calls.  A measurement
- * aid with no counterpart in the reference. */
+ * last call -- ms[0] pre-pass, ms[1] sky-covering kernel, ms[2] sort of the
+ * medium records by coarse cell, ms[3] medium footprint kernel -- and their
+ * number (both may be NULL), then turns event recording on (enable != 0) or
+ * off for the following render calls.  A measurement aid with no
+ * counterpart in the reference. */
 int dmann_timing(dmann_ctx *ctx, int enable, double *ms, int *calls);
 
 /* Writes the finished map, RING order, npix doubles, already divided by
