@@ -17,6 +17,12 @@
 #include "dmann_geom.h"
 #include "dmann_kernels.h"
 
+// resident single-wave blocks per CU of the grouped kernel (extra blocks
+// find the group counter exhausted and exit)
+#ifndef DMANN_GROUP_BPC
+#define DMANN_GROUP_BPC 8
+#endif
+
 using namespace dmann;
 
 struct dmann_ctx {
@@ -181,7 +187,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     if ((e = hipMalloc(&c->d_stats, kNumStats * sizeof(unsigned long long))) != hipSuccess)
         return bail("hipMalloc stats", e);
     c->gc.sort_blocks = dev_cus * 16;
-    c->gc.group_blocks = dev_cus * 8;
+    c->gc.group_blocks = dev_cus * DMANN_GROUP_BPC;
     if ((e = alloc_group(c, 16, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);

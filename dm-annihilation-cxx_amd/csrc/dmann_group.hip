@@ -29,9 +29,15 @@
 
 namespace dmann {
 
-// LDS budget of one wave: 20 KB, so 8 single-wave blocks fit a 160 KB CU.
-constexpr int kGroupCapF32 = 3936; // FP32 window slots
-constexpr int kSeg = 8;            // pixels per segment
+// LDS budget of one wave: 20 KB, so 7 single-wave blocks fit a 160 KB CU.
+#ifndef DMANN_GROUP_CAP
+#define DMANN_GROUP_CAP 3936
+#endif
+#ifndef DMANN_GROUP_SEG
+#define DMANN_GROUP_SEG 8
+#endif
+constexpr int kGroupCapF32 = DMANN_GROUP_CAP; // FP32 window slots
+constexpr int kSeg = DMANN_GROUP_SEG;         // pixels per segment
 
 struct GroupLDS {
     // ring table of up to 64 rings of one member
@@ -93,10 +99,17 @@ __device__ __forceinline__ int ring_table(const PGeo &g, int nside, int iz0, int
         S.ri[lane] = make_int4(rl.kc_rel, cnt, incl - nseg, incl);
         S.rd[lane] = make_int2(rl.col0, rl.startpix);
         if (band_ring0 >= 0) {
+            // slot terms: the band ring owns ncols + kSeg slots from boff;
+            // a segment may fill slots off .. off+len-1 below the limit.
+            // A whole-ring window takes the ring's wrap into the kSeg spare
+            // slots, which needs cnt + kSeg <= nr to keep the masked tail of
+            // the last segment off the first one (else limit 0: direct).
             const int rel = iz0 + lane - band_ring0;
             int off = (rl.col0 - S.bcl[rel]) % rl.nr;
             if (off < 0) off += rl.nr;
-            S.rs[lane] = make_int4(S.boff[rel], S.boff[rel + 1] - S.boff[rel], off, rl.nr);
+            const int ncols = S.boff[rel + 1] - S.boff[rel] - kSeg;
+            const int lim = ncols < rl.nr ? ncols : (cnt + kSeg <= rl.nr ? rl.nr + kSeg : 0);
+            S.rs[lane] = make_int4(S.boff[rel], lim, off, rl.nr);
         }
     } else {
         S.ri[lane] = make_int4(0, 0, incl, incl);
@@ -115,14 +128,38 @@ __device__ __forceinline__ int find_ring(const GroupLDS &S, int s)
     return q;
 }
 
+// Deposits pixels t0 .. t0+len-1 of table ring q straight into the map
+// (global atomics), for bands whose window does not fit the LDS buffer.
+template <int ACC>
+__device__ __forceinline__ void deposit_direct(const GroupLDS &S, int q, int t0, int len, float x0, float step,
+                                               float A, float B, float K4, float sc, const MapRef &map)
+{
+    const int2 d = S.rd[q];
+    const int nr = S.rs[q].w;
+#pragma unroll
+    for (int t = 0; t < kSeg; ++t) {
+        if (t < len) {
+            const float v = weight32<0>(fmaf((float)t, step, x0), A, B, 0.0f, 0.0f, K4) * sc;
+            int col = d.x + t0 + t;
+            if (col < 0) col += nr;
+            else if (col >= nr) col -= nr;
+            acc_add<ACC>(map, d.y + col, v, 0.0);
+        }
+    }
+}
+
 // Walks the table's segments: lane L takes a contiguous run of segments
-// (NS/64 each), 8 pixels per segment with ring terms loaded once per
-// segment.  MODE 0 sums the weights (returns the lane's sum); MODE 1
-// deposits weight * sc into the LDS window (or directly when a pixel falls
-// outside it, which the window construction rules out).
-template <int MODE, int ACC>
-__device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, float K4, float sc, bool direct,
-                                                const MapRef &map)
+// (NS/64 each), kSeg pixels per segment with the ring terms loaded once per
+// segment.  Every lane evaluates all kSeg pixels of its segment; those past
+// the ring's end are masked to zero, so the loop has no divergent branch.
+// MODE 0 sums the weights (returns the lane's sum).  MODE 1 deposits
+// weight * sc: into the LDS window (WINDOW), where each band ring owns
+// window-columns + kSeg slots, so the masked tail of a ring's last segment
+// lands in slots no other lane touches; a segment that would leave its
+// ring's slots (which the window construction rules out) goes to global
+// atomics instead.  Without WINDOW every deposit is a global atomic.
+template <int MODE, int ACC, bool WINDOW>
+__device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, float K4, float sc, const MapRef &map)
 {
     const int s0 = (lane * NS) >> 6, s1 = ((lane + 1) * NS) >> 6;
     const int nmax = (NS + 63) >> 6;
@@ -137,34 +174,29 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
             const int4 r = S.ri[q];
             const int t0 = (s - r.z) * kSeg;
             const int len = r.y - t0 < kSeg ? r.y - t0 : kSeg;
-            float kf = (float)(t0 - r.x);
+            const float x0 = fmaf((float)(t0 - r.x), f.w, f.z);
             if constexpr (MODE == 0) {
 #pragma unroll
                 for (int t = 0; t < kSeg; ++t) {
-                    if (t < len) part += weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, 0.0f, 0.0f, K4);
-                    kf += 1.0f;
+                    const float w = weight32<0>(fmaf((float)t, f.w, x0), f.x, f.y, 0.0f, 0.0f, K4);
+                    part += t < len ? w : 0.0f;
                 }
-            } else {
+            } else if constexpr (WINDOW) {
                 const int4 w = S.rs[q];
                 int off = w.z + t0;
                 if (off >= w.w) off -= w.w;
+                if (off + len <= w.y) {
+                    float *a = &S.acc[w.x + off];
 #pragma unroll
-                for (int t = 0; t < kSeg; ++t) {
-                    if (t < len) {
-                        const float v = weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, 0.0f, 0.0f, K4) * sc;
-                        if (!direct && off < w.y) {
-                            S.acc[w.x + off] += v;
-                        } else {
-                            const int2 d = S.rd[q];
-                            int col = d.x + t0 + t;
-                            if (col < 0) col += w.w;
-                            else if (col >= w.w) col -= w.w;
-                            acc_add<ACC>(map, d.y + col, v, 0.0);
-                        }
+                    for (int t = 0; t < kSeg; ++t) {
+                        const float v = weight32<0>(fmaf((float)t, f.w, x0), f.x, f.y, 0.0f, 0.0f, K4) * sc;
+                        a[t] += t < len ? v : 0.0f;
                     }
-                    kf += 1.0f;
-                    if (++off == w.w) off = 0;
+                } else {
+                    deposit_direct<ACC>(S, q, t0, len, x0, f.w, f.x, f.y, K4, sc, map);
                 }
+            } else {
+                deposit_direct<ACC>(S, q, t0, len, x0, f.w, f.x, f.y, K4, sc, map);
             }
         }
         if ((i & 3) == 3) {
@@ -173,6 +205,58 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
         }
     }
     return total + (double)part;
+}
+
+// Flushes the band's window slots 0 .. Sb-1: one global atomic (form) per
+// non-zero slot.  Slots are taken FB x 64 at a time so that the returning
+// atomics of the compensated form are all in flight before the first
+// result is waited for.
+template <int ACC>
+__device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const MapRef &map)
+{
+    constexpr int FB = 4;
+    int jb = 0;
+    for (int t0 = 0; t0 < Sb; t0 += 64 * FB) {
+        int pix[FB];
+        float v[FB];
+#pragma unroll
+        for (int u = 0; u < FB; ++u) {
+            const int t = t0 + 64 * u + lane;
+            v[u] = 0.0f;
+            pix[u] = 0;
+            if (t < Sb) {
+                while (S.boff[jb + 1] <= t) ++jb;
+                const int nrb = S.bnr[jb];
+                // slot offsets run to ncols + kSeg - 1 (whole rings: nr + kSeg - 1)
+                int col = S.bcl[jb] + (t - S.boff[jb]);
+                if (col >= nrb) col -= nrb;
+                if (col >= nrb) col -= nrb;
+                pix[u] = S.bstart[jb] + col;
+                v[u] = S.acc[t];
+            }
+        }
+        if constexpr (ACC == kAccComp) {
+            // two-float accumulation: hi by a returning atomic, then the
+            // exact rounding error of that addition (TwoSum) into lo
+            float old[FB];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) {
+                old[u] = 0.0f;
+                if (v[u] != 0.0f) old[u] = unsafeAtomicAdd(map.f32 + pix[u], v[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < FB; ++u) {
+                const float sum = old[u] + v[u];
+                const float bb = sum - old[u];
+                const float err = (old[u] - (sum - bb)) + (v[u] - bb);
+                if (err != 0.0f) unsafeAtomicAdd(map.lo + pix[u], err);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < FB; ++u)
+                if (v[u] != 0.0f) acc_add<ACC>(map, pix[u], v[u], 0.0);
+        }
+    }
 }
 
 template <int ACC>
@@ -207,7 +291,7 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
                 const int n = g.nrings - g0 < 64 ? g.nrings - g0 : 64;
                 const int NS = ring_table(g, nside, g.irmin + g0, n, S, lane, -1);
                 T += lane < n ? S.ri[lane].y : 0;
-                nacc += walk_segments<0, ACC>(S, NS, lane, g.K4, 0.0f, false, map);
+                nacc += walk_segments<0, ACC, false>(S, NS, lane, g.K4, 0.0f, map);
                 wave_sync();
             }
             const double norm = wave_sum(nacc);
@@ -263,7 +347,7 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
                     ncols = ch - cl + 1 < nr ? ch - cl + 1 : nr;
                 }
             }
-            const int incl = wave_incl_scan(vr ? ncols : 0, lane);
+            const int incl = wave_incl_scan(vr ? ncols + kSeg : 0, lane);
             const bool fits = vr && incl <= CAP;
             int BH = __popcll(__ballot(fits));
             const bool direct = BH == 0; // one window ring exceeds the buffer
@@ -305,21 +389,12 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
                 g.K4 = __shfl(my_K4, j, 64);
                 const int rlo = b0 > lo ? b0 : lo, rhi = b1 < hi ? b1 : hi;
                 const int NS = ring_table(g, nside, rlo, rhi - rlo + 1, S, lane, b0);
-                walk_segments<1, ACC>(S, NS, lane, g.K4, sc, direct, map);
+                if (direct) walk_segments<1, ACC, false>(S, NS, lane, g.K4, sc, map);
+                else walk_segments<1, ACC, true>(S, NS, lane, g.K4, sc, map);
                 wave_sync();
             }
             if (!direct) {
-                // flush the band: one global atomic (form) per non-zero pixel
-                int jb = 0;
-                for (int t = lane; t < Sb; t += 64) {
-                    while (S.boff[jb + 1] <= t) ++jb;
-                    const int nrb = S.bnr[jb];
-                    int col = S.bcl[jb] + (t - S.boff[jb]);
-                    if (col >= nrb) col -= nrb;
-                    const int pix = S.bstart[jb] + col;
-                    const float v = S.acc[t];
-                    if (v != 0.0f) acc_add<ACC>(map, pix, v, 0.0);
-                }
+                flush_band<ACC>(S, Sb, lane, map);
                 wave_sync();
             }
             b0 += BH;

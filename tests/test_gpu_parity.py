@@ -119,3 +119,43 @@ def test_grouping_sizes(pw, orc, gpu, precision, k):
     print(rep)
     tol, ttol = TOL[precision]
     assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+
+
+def _sky_spread(pw, n, seed, rad=0.06):
+    """n particles at 5 kpc from the observer (8, 0, 0) in seeded random
+    directions, footprint radius 2r = rad."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    d = 5.0
+    a = np.zeros(n, dtype=pw.PARTICLE_DTYPE)
+    a["mass"] = 4100.0
+    a["dens"] = rng.uniform(1e5, 1e7, n)
+    a["hsmooth"] = 0.5 * rad * d
+    pos = np.array([8.0, 0.0, 0.0]) + d * u
+    a["posx"], a["posy"], a["posz"] = pos[:, 0], pos[:, 1], pos[:, 2]
+    return a
+
+
+@pytest.mark.parametrize("precision", ["compensated", "fp32"])
+def test_grouping_wide_windows(pw, orc, gpu, precision):
+    """One group of 60 particles spread over the whole sky (sort-cell nside
+    1, 64 per group): its window spans whole rings (the ring wrap goes to
+    the spare slots) and, at nside 1024, rings wider than the LDS buffer
+    (deposited with global atomics)."""
+    print(f"case wide windows precision={precision}", flush=True)
+    recs = _sky_spread(pw, 60, seed=11)
+    ref, ost = _run_oracle(orc, pw, recs, 1024)
+    p = pw.params_for_observer(1024, precision)
+    with pw.Projector(p, len(recs)) as pr:
+        pr.set_grouping(64, nside_c=1)
+        pr.render(recs)
+        m = pr.get_map()
+        st = pr.stats()
+    _check_counts(st, ost)
+    assert st["large"] < 10
+    rep = rel_err_report(m, ref)
+    print(rep)
+    tol, ttol = TOL[precision]
+    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
