@@ -400,12 +400,18 @@ int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)
     const int cls = prepass_one(P, *reinterpret_cast<const Particle *>(q), rec, bad);
     if (bad || cls != kClsMedium) return -2;
     const PGeo g = particle_geo(rec, P.nside);
+    const RingGeo G = ring_geo(P.nside);
     int64_t mism = 0;
     for (int gi = 0; gi < g.npart; ++gi) {
         const RingRec a = ring_setup(g, P.nside, gi + g.nnorth);
-        const RingLite b = ring_lite(g, P.nside, g.irmin + gi);
-        if (a.cnt != b.cnt || (a.cnt && a.col0 != b.col0) || a.A != b.A || a.B != b.B || a.offm != b.offm ||
-            a.step != b.step || (a.cnt && a.kc_rel != b.kc_rel) || a.startpix != b.startpix || a.nr != b.nr)
+        const RingLite b = ring_lite(g, G, g.irmin + gi);
+        // integer decisions exactly; FP32 terms to a few units of rounding
+        // (the step is an FP32 quotient, the offset a fraction of a step)
+        const bool fl = fabsf(a.A - b.A) <= 1e-7f * a.A && a.B == b.B &&
+                        fabsf(a.step - b.step) <= 2.4e-7f * a.step &&
+                        fabsf(a.offm - b.offm) <= 2.4e-7f * a.step;
+        if (a.cnt != b.cnt || (a.cnt && a.col0 != b.col0) || !fl || (a.cnt && a.kc_rel != b.kc_rel) ||
+            a.startpix != b.startpix || a.nr != b.nr)
             mism++;
     }
     return mism;
@@ -432,6 +438,28 @@ int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int
     }
     const PGeo g = particle_geo(rec, P.nside);
     int64_t k = 0;
+    if (cls == kClsMedium) {
+        // the grouped kernel's per-ring terms and weights
+        const RingGeo G = ring_geo(P.nside);
+        for (int iz = g.irmin; iz <= g.irmax; ++iz) {
+            const RingLite rl = ring_lite(g, G, iz);
+            for (int t = 0; t < rl.cnt; ++t, ++k) {
+                if (k < cap) {
+                    int col = rl.col0 + t;
+                    if (col < 0) col += rl.nr;
+                    else if (col >= rl.nr) col -= rl.nr;
+                    if (pix) pix[k] = rl.startpix + col;
+                    if (w) w[k] = lite_weight32(g, rl, t);
+                }
+            }
+        }
+        if (kind) *kind = k < 2 ? 2 : 3;
+        if (k < 2) {
+            if (cap > 0 && pix) pix[0] = rec.p0;
+            if (cap > 0 && w) w[0] = 1.0f;
+        }
+        return k;
+    }
     for (int gi = 0; gi < g.nrings; ++gi) {
         const RingRec rr = ring_setup(g, P.nside, gi);
         int abase, kw;

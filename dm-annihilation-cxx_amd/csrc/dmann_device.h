@@ -33,6 +33,64 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane)
     return v;
 }
 
+// ---- full-wavefront scans and reductions with DPP moves (row shifts within
+// 16-lane rows, then row broadcasts): no LDS round trips.  Every lane of the
+// wavefront must be active.
+
+struct DppAdd {
+    template <class T> __device__ static T op(T a, T b) { return a + b; }
+};
+struct DppMin {
+    template <class T> __device__ static T op(T a, T b) { return a < b ? a : b; }
+};
+struct DppMax {
+    template <class T> __device__ static T op(T a, T b) { return a > b ? a : b; }
+};
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_move(int v, int ident)
+{
+    return __builtin_amdgcn_update_dpp(ident, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v, double ident)
+{
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(ident), __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(ident), __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Inclusive scan of OP over the 64 lanes; `ident` is OP's identity.
+template <class OP, class T>
+__device__ __forceinline__ T wave_scan_full(T v, T ident)
+{
+    v = OP::op(v, dpp_move<0x111, 0xf>(v, ident)); // row_shr:1
+    v = OP::op(v, dpp_move<0x112, 0xf>(v, ident)); // row_shr:2
+    v = OP::op(v, dpp_move<0x114, 0xf>(v, ident)); // row_shr:4
+    v = OP::op(v, dpp_move<0x118, 0xf>(v, ident)); // row_shr:8
+    v = OP::op(v, dpp_move<0x142, 0xa>(v, ident)); // row_bcast:15 into rows 1, 3
+    v = OP::op(v, dpp_move<0x143, 0xc>(v, ident)); // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+__device__ __forceinline__ int wave_read(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float wave_read(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double wave_read(double v, int lane)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// OP over all 64 lanes, the same value in every lane.
+template <class OP, class T>
+__device__ __forceinline__ T wave_reduce_full(T v, T ident)
+{
+    return wave_read(wave_scan_full<OP>(v, ident), 63);
+}
+
 // Adds one value to the map in the context's accumulation form.
 template <int ACC>
 __device__ __forceinline__ void acc_add(const MapRef &m, int addr, float v32, double v64)

@@ -73,11 +73,26 @@ DMANN_HD int prepass_one(const DevParams &P, const Particle &q, DiscRec &rec, bo
 // Per-particle quantities of the query_disc ring walk (FP64).
 struct PGeo {
     double theta, phi, rad, r, z0, s0, xa, cosr, flux;
+    double dphi; // widest longitude half-width of the disc, asin(sin(rad)/s0) (pi/2 when it reaches a pole)
     int irmin, irmax, nnorth, npart, nsouth, nrings, p0;
     bool all;
     int mode;  // 0: fast polynomials, 1: general (sin / asin)
+    int pv;    // series lengths of the fast form (fast_variant)
     float K4;  // -(0.5/0.333) * 4 * log2(e) / r^2
 };
+
+// Series lengths of the fast weight form for one particle, chosen so that
+// each truncated series stays within 2e-8 relative over its footprint
+// (|x| <= xmax, h <= sin^2(rad/2)):
+//   0: sin^2 2 terms, asin^2 2 terms  (xmax <= 0.0518, rad <= 0.0366)
+//   1: 3 and 3 terms                  (xmax <= 0.2717, rad <= 0.1498)
+//   2: 4 and 6 terms                  (the whole fast geometry)
+DMANN_HD int fast_variant(double rad, double xmax)
+{
+    if (rad <= 0.0366 && xmax <= 0.0518) return 0;
+    if (rad <= 0.1498 && xmax <= 0.2717) return 1;
+    return 2;
+}
 
 DMANN_HD PGeo particle_geo(const DiscRec &rec, int nside)
 {
@@ -108,6 +123,8 @@ DMANN_HD PGeo particle_geo(const DiscRec &rec, int nside)
     const double kc = 0.5 / 0.333; // src/genmap.cpp:143
     g.K4 = (float)(-kc * 4.0 * 1.4426950408889634 / (g.r * g.r));
     g.mode = fast_geometry(g.rad, g.s0) ? 0 : 1;
+    g.dphi = asin(fmin(1.0, sin(g.rad) / g.s0));
+    g.pv = fast_variant(g.rad, g.dphi);
     return g;
 }
 
@@ -201,21 +218,25 @@ DMANN_HD RingRec ring_setup(const PGeo &g, int nside, int gi)
 // the floor() decisions for ip_lo / ip_hi are kept only when the boundary
 // lies farther than a bound on the FP32 error (relative 1e-6 of dphi) from
 // an integer, otherwise dphi is recomputed in FP64 exactly as ring_setup
-// does.  Only the FP32 weight terms are filled (not z, s, shift).
+// does.  Only the FP32 weight terms are filled (not z, s, shift); the
+// column step is an FP32 quotient and the offset of column kc is its
+// fraction of a column (|.| <= 1/2, from the same FP64 product as kc) times
+// the step, so no FP64 division is made.
 struct RingLite {
     int cnt, col0, kc_rel, startpix, nr;
     float A, B, offm, step;
 };
 
-DMANN_HD RingLite ring_lite(const PGeo &g, int nside, int iz)
+DMANN_HD RingLite ring_lite(const PGeo &g, const RingGeo &G, int iz)
 {
     RingLite o;
     int shifted;
-    ring_info(nside, iz, o.startpix, o.nr, shifted);
+    ring_info(G.nside, iz, o.startpix, o.nr, shifted);
     const double shift = shifted ? 0.5 : 0.0;
-    const double z = ring2z(nside, iz);
+    const double z = ring2z(G, iz);
     const double fnr = (double)o.nr;
-    const int kc = (int)floor(fnr * kInvTwoPi * g.phi - shift + 0.5);
+    const double a = fnr * kInvTwoPi * g.phi;
+    const int kc = (int)floor(a - shift + 0.5);
     o.cnt = 0;
     o.col0 = kc;
     const double x = (g.cosr - z * g.z0) * g.xa;
@@ -244,11 +265,10 @@ DMANN_HD RingLite ring_lite(const PGeo &g, int nside, int iz)
     const double s = sqrt((1 - z) * (1 + z));
     double A = (1.0 - (z * g.z0 + s * g.s0)) * 0.5;
     if (A < 0) A = 0;
-    const double step = kTwoPi / fnr;
     o.A = (float)A;
     o.B = (float)(s * g.s0);
-    o.offm = (float)(((double)kc + shift) * step - g.phi);
-    o.step = (float)step;
+    o.step = (float)kTwoPi / (float)o.nr;
+    o.offm = (float)(((double)kc + shift) - a) * o.step;
     o.kc_rel = kc - o.col0;
     return o;
 }
@@ -319,12 +339,60 @@ DMANN_HD float weight32(float x, float A, float B, float Ac, float D, float K4)
     }
 }
 
+// The fast form with the series lengths of fast_variant: PV 2 is
+// weight32<0>; PV 0 and 1 drop the terms below 2e-8 relative.
+template <int PV>
+DMANN_HD float weight_fast(float x, float A, float B, float K4)
+{
+    if constexpr (PV == 2) {
+        return weight32<0>(x, A, B, 0.0f, 0.0f, K4);
+    } else {
+        const float u = x * x;
+        float p = PV == 0 ? -1.0f / 48.0f : fmaf(u, 1.0f / 1440.0f, -1.0f / 48.0f);
+        p = fmaf(u, p, 0.25f);
+        const float h = fmaf(B, u * p, A);
+        // K4 (psi/2)^2 = h (K4 + K4 h/3 [+ K4 8h^2/45])
+        float gq = PV == 0 ? K4 * (1.0f / 3.0f) : fmaf(h, K4 * (8.0f / 45.0f), K4 * (1.0f / 3.0f));
+        gq = fmaf(h, gq, K4);
+        return fast_exp2(h * gq);
+    }
+}
+
+// Pixels per segment of the grouped kernel's ring walks.
+#ifndef DMANN_GROUP_SEG
+#define DMANN_GROUP_SEG 8
+#endif
+constexpr int kSeg = DMANN_GROUP_SEG;
+
+// Longitude offset of pixel t of a RingLite ring as the grouped kernel
+// forms it: from the first pixel of t's segment, then t's place in it.
+DMANN_HD float seg_x(const RingLite &rl, int t)
+{
+    const int t0 = t / kSeg * kSeg;
+    const float x0 = fmaf((float)(t0 - rl.kc_rel), rl.step, rl.offm);
+    return fmaf((float)(t - t0), rl.step, x0);
+}
+
+DMANN_HD float lite_weight32(const PGeo &g, const RingLite &rl, int t)
+{
+    const float x = seg_x(rl, t);
+    switch (g.pv) {
+    case 0: return weight_fast<0>(x, rl.A, rl.B, g.K4);
+    case 1: return weight_fast<1>(x, rl.A, rl.B, g.K4);
+    default: return weight_fast<2>(x, rl.A, rl.B, g.K4);
+    }
+}
+
 DMANN_HD float ring_weight32(const PGeo &g, const RingRec &rr, int t)
 {
     const float kf = (float)(t - rr.kc_rel);
     const float x = fmaf(kf, rr.step, rr.offm);
-    return g.mode == 0 ? weight32<0>(x, rr.A, rr.B, 0.0f, 0.0f, g.K4)
-                       : weight32<1>(x, rr.A, rr.E, rr.Ac, rr.D, g.K4);
+    if (g.mode != 0) return weight32<1>(x, rr.A, rr.E, rr.Ac, rr.D, g.K4);
+    switch (g.pv) {
+    case 0: return weight_fast<0>(x, rr.A, rr.B, g.K4);
+    case 1: return weight_fast<1>(x, rr.A, rr.B, g.K4);
+    default: return weight_fast<2>(x, rr.A, rr.B, g.K4);
+    }
 }
 
 } // namespace dmann

@@ -37,17 +37,34 @@ DMANN_HD int ring_above(int nside, double z)
     return (z > 0) ? iring : 4 * nside - iring - 1;
 }
 
-// z = cos(theta) of the pixel centres of ring `ring`.
-DMANN_HD double ring2z(int nside, int ring)
+// Factors of ring2z for one nside, computed once per kernel.
+struct RingGeo {
+    int nside;
+    double fact1, fact2;
+};
+
+DMANN_HD RingGeo ring_geo(int nside)
 {
-    double fact2 = 4. / (double)npix_of(nside);
-    double fact1 = (double)(2 * nside) * fact2;
+    RingGeo G;
+    G.nside = nside;
+    G.fact2 = 4. / (double)npix_of(nside);
+    G.fact1 = (double)(2 * nside) * G.fact2;
+    return G;
+}
+
+// z = cos(theta) of the pixel centres of ring `ring`.
+DMANN_HD double ring2z(const RingGeo &G, int ring)
+{
+    const int nside = G.nside;
+    const double fact1 = G.fact1, fact2 = G.fact2;
     int64_t r = ring;
     if (r < nside) return 1 - (double)(r * r) * fact2;
     if (r <= 3 * nside) return (double)(2 * nside - r) * fact1;
     r = 4 * nside - r;
     return (double)(r * r) * fact2 - 1;
 }
+
+DMANN_HD double ring2z(int nside, int ring) { return ring2z(ring_geo(nside), ring); }
 
 // First pixel, pixel count and half-pixel phase shift of a ring.
 DMANN_HD void ring_info(int nside, int ring, int &startpix, int &ringpix, int &shifted)

@@ -29,15 +29,11 @@
 
 namespace dmann {
 
-// LDS budget of one wave: 20 KB, so 7 single-wave blocks fit a 160 KB CU.
+// LDS budget of one wave: 20 KB, so 8 single-wave blocks fit a 160 KB CU.
 #ifndef DMANN_GROUP_CAP
-#define DMANN_GROUP_CAP 3936
-#endif
-#ifndef DMANN_GROUP_SEG
-#define DMANN_GROUP_SEG 8
+#define DMANN_GROUP_CAP 3760
 #endif
 constexpr int kGroupCapF32 = DMANN_GROUP_CAP; // FP32 window slots
-constexpr int kSeg = DMANN_GROUP_SEG;         // pixels per segment
 
 struct GroupLDS {
     // ring table of up to 64 rings of one member
@@ -45,55 +41,46 @@ struct GroupLDS {
     int4 ri[64];     // kc_rel, cnt, first segment, end segment
     int4 rs[64];     // window slot offset, window columns, column offset in the window ring, nr
     int2 rd[64];     // col0, first pixel of the ring (direct deposits)
-    // band table
-    int boff[68];    // band slot prefix (boff[j]..boff[j+1] for band ring j)
-    int bcl[64];     // band ring: first window column, wrapped into [0, nr)
-    int bnr[64];     // band ring: pixels in ring
-    int bstart[64];  // band ring: first pixel index
+    // band table: ring j owns slots bt[j].x .. bt[j+1].x - 1 (a multiple
+    // of kSeg, at least window columns + kSeg)
+    int4 bt[65];     // first slot, first window column (wrapped into [0, nr)), nr, first pixel
+    int bnc[64];     // window columns
+    unsigned char ros[kGroupCapF32 / kSeg]; // band ring owning each block of kSeg slots
+    unsigned char q0[64];                   // table ring of each lane's first segment
     float acc[kGroupCapF32];
 };
+static_assert(kGroupCapF32 % kSeg == 0, "window blocks");
+#ifndef DMANN_GROUP_LDS_MAX
+#define DMANN_GROUP_LDS_MAX 20480
+#endif
+static_assert(sizeof(GroupLDS) <= DMANN_GROUP_LDS_MAX, "eight waves per CU");
 
-__device__ __forceinline__ double wave_min_d(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_max_d(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_min_i(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max_i(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
+__device__ __forceinline__ double wave_min_d(double v) { return wave_reduce_full<DppMin>(v, 1e300); }
+__device__ __forceinline__ double wave_max_d(double v) { return wave_reduce_full<DppMax>(v, -1e300); }
+__device__ __forceinline__ int wave_min_i(int v) { return wave_reduce_full<DppMin>(v, 0x7fffffff); }
+__device__ __forceinline__ int wave_max_i(int v) { return wave_reduce_full<DppMax>(v, -0x7fffffff - 1); }
 
 // Sets up the ring table for rings iz0 .. iz0+n-1 (n <= 64) of a member:
 // lane q handles ring iz0 + q.  With band_ring0 >= 0 the window slot terms
 // of the band are filled too.  Returns the number of segments.
-__device__ __forceinline__ int ring_table(const PGeo &g, int nside, int iz0, int n, GroupLDS &S, int lane,
+__device__ __forceinline__ int ring_table(const PGeo &g, const RingGeo &G, int iz0, int n, GroupLDS &S, int lane,
                                           int band_ring0)
 {
     const bool on = lane < n;
     RingLite rl;
     int cnt = 0;
     if (on) {
-        rl = ring_lite(g, nside, iz0 + lane);
+        rl = ring_lite(g, G, iz0 + lane);
         cnt = rl.cnt;
     }
     const int nseg = (cnt + kSeg - 1) / kSeg;
-    const int incl = wave_incl_scan(nseg, lane);
-    const int NS = __shfl(incl, 63, 64);
+    const int incl = wave_scan_full<DppAdd>(nseg, 0);
+    const int NS = wave_read(incl, 63);
+    // lanes whose first segment (lane NS / 64) lies in this ring's range
+    if (nseg > 0) {
+        const int llo = ((incl - nseg) * 64 + NS - 1) / NS, lhi = (incl * 64 + NS - 1) / NS;
+        for (int l = llo; l < lhi; ++l) S.q0[l] = (unsigned char)lane;
+    }
     if (on) {
         S.rf[lane] = make_float4(rl.A, rl.B, rl.offm, rl.step);
         S.ri[lane] = make_int4(rl.kc_rel, cnt, incl - nseg, incl);
@@ -105,11 +92,12 @@ __device__ __forceinline__ int ring_table(const PGeo &g, int nside, int iz0, int
             // slots, which needs cnt + kSeg <= nr to keep the masked tail of
             // the last segment off the first one (else limit 0: direct).
             const int rel = iz0 + lane - band_ring0;
-            int off = (rl.col0 - S.bcl[rel]) % rl.nr;
+            const int4 b = S.bt[rel];
+            int off = (rl.col0 - b.y) % rl.nr;
             if (off < 0) off += rl.nr;
-            const int ncols = S.boff[rel + 1] - S.boff[rel] - kSeg;
+            const int ncols = S.bnc[rel];
             const int lim = ncols < rl.nr ? ncols : (cnt + kSeg <= rl.nr ? rl.nr + kSeg : 0);
-            S.rs[lane] = make_int4(S.boff[rel], lim, off, rl.nr);
+            S.rs[lane] = make_int4(b.x, lim, off, rl.nr);
         }
     } else {
         S.ri[lane] = make_int4(0, 0, incl, incl);
@@ -118,19 +106,9 @@ __device__ __forceinline__ int ring_table(const PGeo &g, int nside, int iz0, int
     return NS;
 }
 
-// First ring q (of n) whose segment range contains segment s.
-__device__ __forceinline__ int find_ring(const GroupLDS &S, int s)
-{
-    int q = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1)
-        if (S.ri[q + step - 1].w <= s) q += step;
-    return q;
-}
-
 // Deposits pixels t0 .. t0+len-1 of table ring q straight into the map
 // (global atomics), for bands whose window does not fit the LDS buffer.
-template <int ACC>
+template <int ACC, int PV>
 __device__ __forceinline__ void deposit_direct(const GroupLDS &S, int q, int t0, int len, float x0, float step,
                                                float A, float B, float K4, float sc, const MapRef &map)
 {
@@ -139,7 +117,7 @@ __device__ __forceinline__ void deposit_direct(const GroupLDS &S, int q, int t0,
 #pragma unroll
     for (int t = 0; t < kSeg; ++t) {
         if (t < len) {
-            const float v = weight32<0>(fmaf((float)t, step, x0), A, B, 0.0f, 0.0f, K4) * sc;
+            const float v = weight_fast<PV>(fmaf((float)t, step, x0), A, B, K4) * sc;
             int col = d.x + t0 + t;
             if (col < 0) col += nr;
             else if (col >= nr) col -= nr;
@@ -158,14 +136,14 @@ __device__ __forceinline__ void deposit_direct(const GroupLDS &S, int q, int t0,
 // lands in slots no other lane touches; a segment that would leave its
 // ring's slots (which the window construction rules out) goes to global
 // atomics instead.  Without WINDOW every deposit is a global atomic.
-template <int MODE, int ACC, bool WINDOW>
+template <int MODE, int ACC, bool WINDOW, int PV>
 __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, float K4, float sc, const MapRef &map)
 {
     const int s0 = (lane * NS) >> 6, s1 = ((lane + 1) * NS) >> 6;
     const int nmax = (NS + 63) >> 6;
     double total = 0.0;
     float part = 0.0f;
-    int q = s0 < s1 ? find_ring(S, s0) : 0;
+    int q = s0 < s1 ? S.q0[lane] : 0;
     for (int i = 0; i < nmax; ++i) {
         const int s = s0 + i;
         if (s < s1) {
@@ -178,7 +156,7 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
             if constexpr (MODE == 0) {
 #pragma unroll
                 for (int t = 0; t < kSeg; ++t) {
-                    const float w = weight32<0>(fmaf((float)t, f.w, x0), f.x, f.y, 0.0f, 0.0f, K4);
+                    const float w = weight_fast<PV>(fmaf((float)t, f.w, x0), f.x, f.y, K4);
                     part += t < len ? w : 0.0f;
                 }
             } else if constexpr (WINDOW) {
@@ -189,14 +167,14 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
                     float *a = &S.acc[w.x + off];
 #pragma unroll
                     for (int t = 0; t < kSeg; ++t) {
-                        const float v = weight32<0>(fmaf((float)t, f.w, x0), f.x, f.y, 0.0f, 0.0f, K4) * sc;
+                        const float v = weight_fast<PV>(fmaf((float)t, f.w, x0), f.x, f.y, K4) * sc;
                         a[t] += t < len ? v : 0.0f;
                     }
                 } else {
-                    deposit_direct<ACC>(S, q, t0, len, x0, f.w, f.x, f.y, K4, sc, map);
+                    deposit_direct<ACC, PV>(S, q, t0, len, x0, f.w, f.x, f.y, K4, sc, map);
                 }
             } else {
-                deposit_direct<ACC>(S, q, t0, len, x0, f.w, f.x, f.y, K4, sc, map);
+                deposit_direct<ACC, PV>(S, q, t0, len, x0, f.w, f.x, f.y, K4, sc, map);
             }
         }
         if ((i & 3) == 3) {
@@ -207,33 +185,46 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
     return total + (double)part;
 }
 
-// Flushes the band's window slots 0 .. Sb-1: one global atomic (form) per
-// non-zero slot.  Slots are taken FB x 64 at a time so that the returning
-// atomics of the compensated form are all in flight before the first
-// result is waited for.
+// walk_segments with the particle's series lengths (fast_variant).
+template <int MODE, int ACC, bool WINDOW>
+__device__ __forceinline__ double walk_pv(int pv, GroupLDS &S, int NS, int lane, float K4, float sc, const MapRef &map)
+{
+    switch (pv) {
+    case 0: return walk_segments<MODE, ACC, WINDOW, 0>(S, NS, lane, K4, sc, map);
+    case 1: return walk_segments<MODE, ACC, WINDOW, 1>(S, NS, lane, K4, sc, map);
+    default: return walk_segments<MODE, ACC, WINDOW, 2>(S, NS, lane, K4, sc, map);
+    }
+}
+
+// Flushes the band's window slots 0 .. Sb-1 (Sb >= 1): one global atomic
+// (form) per non-zero slot.  Slots are taken FB x 64 at a time with
+// unconditional (clamped) LDS reads, so the lookups of the FB chunks overlap
+// and the returning atomics of the compensated form are all in flight
+// before the first result is waited for.
 template <int ACC>
-__device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const MapRef &map)
+__device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const MapRef &map, unsigned npix)
 {
     constexpr int FB = 4;
-    int jb = 0;
     for (int t0 = 0; t0 < Sb; t0 += 64 * FB) {
-        int pix[FB];
+        int tc[FB], j[FB], pix[FB];
         float v[FB];
 #pragma unroll
         for (int u = 0; u < FB; ++u) {
             const int t = t0 + 64 * u + lane;
-            v[u] = 0.0f;
-            pix[u] = 0;
-            if (t < Sb) {
-                while (S.boff[jb + 1] <= t) ++jb;
-                const int nrb = S.bnr[jb];
-                // slot offsets run to ncols + kSeg - 1 (whole rings: nr + kSeg - 1)
-                int col = S.bcl[jb] + (t - S.boff[jb]);
-                if (col >= nrb) col -= nrb;
-                if (col >= nrb) col -= nrb;
-                pix[u] = S.bstart[jb] + col;
-                v[u] = S.acc[t];
-            }
+            tc[u] = t < Sb ? t : Sb - 1;
+            j[u] = S.ros[(unsigned)tc[u] / kSeg];
+        }
+#pragma unroll
+        for (int u = 0; u < FB; ++u) {
+            const int4 b = S.bt[j[u]];
+            const float a = S.acc[tc[u]];
+            // slot offsets run below ncols + 2 kSeg (whole rings: nr + 2 kSeg)
+            int col = b.y + (tc[u] - b.x);
+            if (col >= b.z) col -= b.z;
+            if (col >= b.z) col -= b.z;
+            pix[u] = b.w + col;
+            // (the pixel test only guards the map against a malformed table)
+            v[u] = t0 + 64 * u + lane < Sb && (unsigned)pix[u] < npix ? a : 0.0f;
         }
         if constexpr (ACC == kAccComp) {
             // two-float accumulation: hi by a returning atomic, then the
@@ -259,14 +250,22 @@ __device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const 
     }
 }
 
+#ifdef DMANN_GROUP_WPE
+#define DMANN_GROUP_ATTR __attribute__((amdgpu_waves_per_eu(DMANN_GROUP_WPE)))
+#else
+#define DMANN_GROUP_ATTR
+#endif
+
 template <int ACC>
-__global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ recs, const unsigned *__restrict__ idx,
+__global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRec *__restrict__ recs, const unsigned *__restrict__ idx,
                                                    const unsigned *__restrict__ counts, int nside, int K, MapRef map,
                                                    unsigned long long *__restrict__ stats, unsigned *__restrict__ work)
 {
     __shared__ GroupLDS S;
     constexpr int CAP = kGroupCapF32;
     const int lane = threadIdx.x;
+    const RingGeo G = ring_geo(nside);
+    const unsigned npix = (unsigned)npix_of(nside);
     const unsigned count = counts[0];
     const unsigned ngroups = (count + (unsigned)K - 1u) / (unsigned)K;
     unsigned long long n_ext = 0, n_fb = 0, n_contrib = 0;
@@ -274,45 +273,64 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
     for (;;) {
         unsigned grp = 0;
         if (lane == 0) grp = atomicAdd(work, 1u);
-        grp = __shfl(grp, 0, 64);
+        grp = (unsigned)wave_read((int)grp, 0);
         if (grp >= ngroups) break;
         const unsigned m0 = grp * (unsigned)K;
         const int nm = (int)((m0 + (unsigned)K < count ? m0 + (unsigned)K : count) - m0);
         // ---- phase 1: normalisation of every member
+        // member geometry, one member per lane
         float my_scale = 0.0f, my_K4 = 0.0f;
+        int my_pv = 2, my_p0 = 0;
         int my_lo = 0x7fffffff, my_hi = -0x7fffffff;
-        double my_phi = 0.0, my_dphi = 0.0, my_z0 = 0.0, my_s0 = 0.0, my_xa = 0.0, my_cosr = 0.0;
+        double my_phi = 0.0, my_dphi = 0.0, my_z0 = 0.0, my_s0 = 0.0, my_xa = 0.0, my_cosr = 0.0, my_flux = 0.0;
         bool my_on = false;
+        if (lane < nm) {
+            const PGeo mg = particle_geo(recs[idx[m0 + lane]], nside);
+            my_lo = mg.irmin;
+            my_hi = mg.irmax;
+            my_phi = mg.phi;
+            my_dphi = mg.dphi;
+            my_pv = mg.pv;
+            my_z0 = mg.z0;
+            my_s0 = mg.s0;
+            my_xa = mg.xa;
+            my_cosr = mg.cosr;
+            my_K4 = mg.K4;
+            my_flux = mg.flux;
+            my_p0 = mg.p0;
+        }
         for (int j = 0; j < nm; ++j) {
-            const PGeo g = particle_geo(recs[idx[m0 + j]], nside);
+            PGeo g;
+            g.phi = wave_read(my_phi, j);
+            g.z0 = wave_read(my_z0, j);
+            g.s0 = wave_read(my_s0, j);
+            g.xa = wave_read(my_xa, j);
+            g.cosr = wave_read(my_cosr, j);
+            g.K4 = wave_read(my_K4, j);
+            g.pv = wave_read(my_pv, j);
+            g.irmin = wave_read(my_lo, j);
+            g.irmax = wave_read(my_hi, j);
+            // medium-class discs reach no pole: only partial rings
+            g.nrings = g.irmax >= g.irmin ? g.irmax - g.irmin + 1 : 0;
             double nacc = 0.0;
             int T = 0;
             for (int g0 = 0; g0 < g.nrings; g0 += 64) {
                 const int n = g.nrings - g0 < 64 ? g.nrings - g0 : 64;
-                const int NS = ring_table(g, nside, g.irmin + g0, n, S, lane, -1);
+                const int NS = ring_table(g, G, g.irmin + g0, n, S, lane, -1);
                 T += lane < n ? S.ri[lane].y : 0;
-                nacc += walk_segments<0, ACC, false>(S, NS, lane, g.K4, 0.0f, map);
+                nacc += walk_pv<0, ACC, false>(g.pv, S, NS, lane, g.K4, 0.0f, map);
                 wave_sync();
             }
-            const double norm = wave_sum(nacc);
-            T = (int)wave_sum((double)T);
+            const double norm = wave_reduce_full<DppAdd>(nacc, 0.0);
+            T = wave_reduce_full<DppAdd>(T, 0);
             if (T < 2) {
-                if (lane == 0) acc_point<ACC>(map, g.p0, g.flux);
+                if (lane == j) acc_point<ACC>(map, my_p0, my_flux);
                 n_fb++;
             } else {
                 n_ext++;
                 n_contrib += (unsigned long long)T;
                 if (lane == j) {
-                    my_scale = (float)(g.flux / norm);
-                    my_lo = g.irmin;
-                    my_hi = g.irmax;
-                    my_phi = g.phi;
-                    my_dphi = asin(fmin(1.0, sin(g.rad) / g.s0));
-                    my_z0 = g.z0;
-                    my_s0 = g.s0;
-                    my_xa = g.xa;
-                    my_cosr = g.cosr;
-                    my_K4 = g.K4;
+                    my_scale = (float)(my_flux / norm);
                     my_on = true;
                 }
             }
@@ -323,7 +341,7 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
         if (R0 > R1) continue;
         const uint64_t onmask = __ballot(my_on);
         const int lead = __ffsll((unsigned long long)onmask) - 1;
-        const double phic = __shfl(my_phi, lead, 64);
+        const double phic = wave_read(my_phi, lead);
         double dl = my_phi - phic;
         if (dl > kPi) dl -= kTwoPi;
         if (dl < -kPi) dl += kTwoPi;
@@ -347,54 +365,54 @@ __global__ __launch_bounds__(64) void k_disc_group(const DiscRec *__restrict__ r
                     ncols = ch - cl + 1 < nr ? ch - cl + 1 : nr;
                 }
             }
-            const int incl = wave_incl_scan(vr ? ncols + kSeg : 0, lane);
+            const int alloc = vr ? (ncols + 2 * kSeg - 1) / kSeg * kSeg : 0;
+            const int incl = wave_scan_full<DppAdd>(alloc, 0);
             const bool fits = vr && incl <= CAP;
             int BH = __popcll(__ballot(fits));
             const bool direct = BH == 0; // one window ring exceeds the buffer
             if (direct) BH = (R1 - b0 + 1) < 64 ? (R1 - b0 + 1) : 64;
             int Sb = 0;
             if (!direct) {
+                Sb = wave_read(incl, BH - 1);
                 if (lane < BH) {
                     int clw = cl % nr;
                     if (clw < 0) clw += nr;
-                    S.boff[lane + 1] = incl;
-                    S.bcl[lane] = clw;
-                    S.bnr[lane] = nr;
-                    S.bstart[lane] = startpix;
+                    S.bt[lane] = make_int4(incl - alloc, clw, nr, startpix);
+                    S.bnc[lane] = ncols;
+                    for (int k = (incl - alloc) / kSeg; k < incl / kSeg; ++k) S.ros[k] = (unsigned char)lane;
                 }
-                if (lane == 0) S.boff[0] = 0;
-                Sb = __shfl(incl, BH - 1, 64);
+                if (lane == 0) S.bt[BH] = make_int4(Sb, 0, 1, 0);
                 for (int t = lane; t < Sb; t += 64) S.acc[t] = 0.0f;
             } else {
                 // direct deposits still need the band table for slot terms
                 if (lane < BH) {
-                    S.boff[lane + 1] = 0;
-                    S.bcl[lane] = 0;
+                    S.bt[lane] = make_int4(0, 0, nr, startpix);
+                    S.bnc[lane] = 0;
                 }
-                if (lane == 0) S.boff[0] = 0;
             }
             wave_sync();
             const int b1 = b0 + BH - 1;
             for (int j = 0; j < nm; ++j) {
-                const int lo = __shfl(my_lo, j, 64), hi = __shfl(my_hi, j, 64);
-                if (!__shfl((int)my_on, j, 64) || hi < b0 || lo > b1) continue;
-                const float sc = __shfl(my_scale, j, 64);
+                const int lo = wave_read(my_lo, j), hi = wave_read(my_hi, j);
+                if (!wave_read((int)my_on, j) || hi < b0 || lo > b1) continue;
+                const float sc = wave_read(my_scale, j);
                 // the member's ring-walk terms, from its lane (computed in phase 1)
                 PGeo g;
-                g.phi = __shfl(my_phi, j, 64);
-                g.z0 = __shfl(my_z0, j, 64);
-                g.s0 = __shfl(my_s0, j, 64);
-                g.xa = __shfl(my_xa, j, 64);
-                g.cosr = __shfl(my_cosr, j, 64);
-                g.K4 = __shfl(my_K4, j, 64);
+                g.phi = wave_read(my_phi, j);
+                g.z0 = wave_read(my_z0, j);
+                g.s0 = wave_read(my_s0, j);
+                g.xa = wave_read(my_xa, j);
+                g.cosr = wave_read(my_cosr, j);
+                g.K4 = wave_read(my_K4, j);
                 const int rlo = b0 > lo ? b0 : lo, rhi = b1 < hi ? b1 : hi;
-                const int NS = ring_table(g, nside, rlo, rhi - rlo + 1, S, lane, b0);
-                if (direct) walk_segments<1, ACC, false>(S, NS, lane, g.K4, sc, map);
-                else walk_segments<1, ACC, true>(S, NS, lane, g.K4, sc, map);
+                const int NS = ring_table(g, G, rlo, rhi - rlo + 1, S, lane, b0);
+                const int pv = wave_read(my_pv, j);
+                if (direct) walk_pv<1, ACC, false>(pv, S, NS, lane, g.K4, sc, map);
+                else walk_pv<1, ACC, true>(pv, S, NS, lane, g.K4, sc, map);
                 wave_sync();
             }
             if (!direct) {
-                flush_band<ACC>(S, Sb, lane, map);
+                flush_band<ACC>(S, Sb, lane, map, npix);
                 wave_sync();
             }
             b0 += BH;

@@ -207,8 +207,10 @@ int dmann_write_healpix_fits(const char *path, const double *map, int64_t npix, 
 int dmann_read_healpix_fits(const char *path, double *map, int64_t cap, int *nside, int64_t *npix);
 
 /* Host-side footprint of one particle, computed with the same code the
- * kernels run (no GPU involved): pixels (RING) and unnormalised FP32 weights
- * of the particle's footprint in the kernels' ring order, up to cap entries.
+ * kernels run (no GPU involved; medium-class particles with the grouped
+ * kernel's ring terms and weight series, the others with the sky-covering
+ * kernel's): pixels (RING) and unnormalised FP32 weights of the particle's
+ * footprint in the kernels' ring order, up to cap entries.
  * *kind: 0 filtered, 1 sub-pixel (whole flux to pix[0]), 2 footprint of < 2
  * pixels (whole flux to pix[0]), 3 spread.  Returns the footprint size (1
  * for kinds 1 and 2), 0 when filtered, < 0 on bad input or an undefined
@@ -218,9 +220,11 @@ int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int
 
 /* Host-side check of the medium-class ring walk: the number of rings of a
  * medium-class particle where the fast ring setup (FP32 half-width with an
- * FP64 recomputation near column boundaries) differs from the exact FP64
- * setup (0 expected); -2 when the particle is not in the medium class.  A
- * test aid with no counterpart in the reference. */
+ * FP64 recomputation near column boundaries, FP32 column step) differs from
+ * the exact FP64 setup -- in a column decision, or in an FP32 weight term by
+ * more than a few units of rounding (0 expected); -2 when the particle is
+ * not in the medium class.  A test aid with no counterpart in the
+ * reference. */
 int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q);
 
 /* Message of the last failed call on ctx (or of the last failed
