@@ -77,7 +77,11 @@ static void free_group(dmann_ctx *c)
     (void)hipFree(c->gc.hist);
     (void)hipFree(c->gc.sums);
     (void)hipFree(c->gc.work);
+    (void)hipFree(c->gc.tab_f);
+    (void)hipFree(c->gc.tab_i);
     c->gc.keys = c->gc.idx = c->gc.hist = c->gc.sums = c->gc.work = nullptr;
+    c->gc.tab_f = nullptr;
+    c->gc.tab_i = nullptr;
 }
 
 // Sort-cell buffers for grouped rendering (k > 0); k == 0 frees them.
@@ -98,6 +102,9 @@ static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
     if ((e = hipMalloc(&c->gc.hist, nbins * sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMalloc(&c->gc.sums, ((nbins + 1023) / 1024) * sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMalloc(&c->gc.work, sizeof(unsigned))) != hipSuccess) return e;
+    const size_t rows = (size_t)c->gc.group_blocks * (size_t)k * 64;
+    if ((e = hipMalloc(&c->gc.tab_f, rows * sizeof(float4))) != hipSuccess) return e;
+    if ((e = hipMalloc(&c->gc.tab_i, rows * sizeof(int4))) != hipSuccess) return e;
     return hipSuccess;
 }
 

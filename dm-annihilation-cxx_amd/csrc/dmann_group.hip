@@ -60,25 +60,53 @@ __device__ __forceinline__ double wave_max_d(double v) { return wave_reduce_full
 __device__ __forceinline__ int wave_min_i(int v) { return wave_reduce_full<DppMin>(v, 0x7fffffff); }
 __device__ __forceinline__ int wave_max_i(int v) { return wave_reduce_full<DppMax>(v, -0x7fffffff - 1); }
 
-// Sets up the ring table for rings iz0 .. iz0+n-1 (n <= 64) of a member:
-// lane q handles ring iz0 + q.  With band_ring0 >= 0 the window slot terms
-// of the band are filled too.  Returns the number of segments.
-__device__ __forceinline__ int ring_table(const PGeo &g, const RingGeo &G, int iz0, int n, GroupLDS &S, int lane,
-                                          int band_ring0)
+// Ring terms packed for the per-wave cache: (A, B, offm, step) and
+// (cnt, col0, kc_rel in the low / nr in the high 16 bits, startpix).
+__device__ __forceinline__ void ring_pack(const RingLite &rl, float4 &f, int4 &i)
 {
-    const bool on = lane < n;
+    f = make_float4(rl.A, rl.B, rl.offm, rl.step);
+    i = make_int4(rl.cnt, rl.col0, (int)(((unsigned)rl.nr << 16) | ((unsigned)rl.kc_rel & 0xffffu)), rl.startpix);
+}
+__device__ __forceinline__ RingLite ring_unpack(const float4 &f, const int4 &i)
+{
     RingLite rl;
-    int cnt = 0;
-    if (on) {
-        rl = ring_lite(g, G, iz0 + lane);
-        cnt = rl.cnt;
-    }
+    rl.A = f.x;
+    rl.B = f.y;
+    rl.offm = f.z;
+    rl.step = f.w;
+    rl.cnt = i.x;
+    rl.col0 = i.y;
+    rl.nr = (int)((unsigned)i.z >> 16);
+    rl.kc_rel = (int)(short)(i.z & 0xffff);
+    rl.startpix = i.w;
+    return rl;
+}
+
+// ceil(a / b) for 0 <= a < 2^24, 0 < b, from an FP32 quotient corrected by
+// one step either way.
+__device__ __forceinline__ int ceil_div_small(int a, int b, float rb)
+{
+    int q = (int)((float)a * rb);
+    if (q * b < a) ++q;
+    if (q > 0 && (q - 1) * b >= a) --q;
+    return q;
+}
+
+// Fills the LDS ring table from lane q's ring terms (lanes q < n hold ring
+// iz0 + q): segment ranges, the table ring of every lane's first segment
+// and, with band_ring0 >= 0, the window slot terms of the band.  Returns
+// the number of segments.  Every lane must call it.
+__device__ __forceinline__ int ring_table_fill(const RingLite &rl, bool on, int iz0, GroupLDS &S, int lane,
+                                               int band_ring0)
+{
+    const int cnt = on ? rl.cnt : 0;
     const int nseg = (cnt + kSeg - 1) / kSeg;
     const int incl = wave_scan_full<DppAdd>(nseg, 0);
     const int NS = wave_read(incl, 63);
     // lanes whose first segment (lane NS / 64) lies in this ring's range
     if (nseg > 0) {
-        const int llo = ((incl - nseg) * 64 + NS - 1) / NS, lhi = (incl * 64 + NS - 1) / NS;
+        const float rns = 1.0f / (float)NS;
+        const int llo = ceil_div_small((incl - nseg) * 64, NS, rns), lhi = ceil_div_small(incl * 64, NS, rns);
         for (int l = llo; l < lhi; ++l) S.q0[l] = (unsigned char)lane;
     }
     if (on) {
@@ -93,7 +121,9 @@ __device__ __forceinline__ int ring_table(const PGeo &g, const RingGeo &G, int i
             // the last segment off the first one (else limit 0: direct).
             const int rel = iz0 + lane - band_ring0;
             const int4 b = S.bt[rel];
-            int off = (rl.col0 - b.y) % rl.nr;
+            // col0 in [-nr/2 - 1, nr), first window column in [0, nr)
+            int off = rl.col0 - b.y;
+            if (off < 0) off += rl.nr;
             if (off < 0) off += rl.nr;
             const int ncols = S.bnc[rel];
             const int lim = ncols < rl.nr ? ncols : (cnt + kSeg <= rl.nr ? rl.nr + kSeg : 0);
@@ -104,6 +134,28 @@ __device__ __forceinline__ int ring_table(const PGeo &g, const RingGeo &G, int i
     }
     wave_sync();
     return NS;
+}
+
+// Sets up the ring table for rings iz0 .. iz0+n-1 (n <= 64) of a member:
+// lane q computes ring iz0 + q (and stores its terms to cache[q] when a
+// cache row is given).  Returns the number of segments.
+__device__ __forceinline__ int ring_table(const PGeo &g, const RingGeo &G, int iz0, int n, GroupLDS &S, int lane,
+                                          int band_ring0, float4 *cache_f = nullptr, int4 *cache_i = nullptr)
+{
+    const bool on = lane < n;
+    RingLite rl;
+    rl.cnt = 0;
+    if (on) {
+        rl = ring_lite(g, G, iz0 + lane);
+        if (cache_f) {
+            float4 f;
+            int4 i;
+            ring_pack(rl, f, i);
+            cache_f[lane] = f;
+            cache_i[lane] = i;
+        }
+    }
+    return ring_table_fill(rl, on, iz0, S, lane, band_ring0);
 }
 
 // Deposits pixels t0 .. t0+len-1 of table ring q straight into the map
@@ -204,7 +256,7 @@ __device__ __forceinline__ double walk_pv(int pv, GroupLDS &S, int NS, int lane,
 template <int ACC>
 __device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const MapRef &map, unsigned npix)
 {
-    constexpr int FB = 4;
+    constexpr int FB = ACC == kAccComp ? 8 : 4;
     for (int t0 = 0; t0 < Sb; t0 += 64 * FB) {
         int tc[FB], j[FB], pix[FB];
         float v[FB];
@@ -259,7 +311,8 @@ __device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const 
 template <int ACC>
 __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRec *__restrict__ recs, const unsigned *__restrict__ idx,
                                                    const unsigned *__restrict__ counts, int nside, int K, MapRef map,
-                                                   unsigned long long *__restrict__ stats, unsigned *__restrict__ work)
+                                                   unsigned long long *__restrict__ stats, unsigned *__restrict__ work,
+                                                   float4 *__restrict__ tab_f, int4 *__restrict__ tab_i)
 {
     __shared__ GroupLDS S;
     constexpr int CAP = kGroupCapF32;
@@ -316,7 +369,10 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
             int T = 0;
             for (int g0 = 0; g0 < g.nrings; g0 += 64) {
                 const int n = g.nrings - g0 < 64 ? g.nrings - g0 : 64;
-                const int NS = ring_table(g, G, g.irmin + g0, n, S, lane, -1);
+                // the first 64 rings go to the wave's cache row of member j
+                const size_t row = ((size_t)blockIdx.x * (size_t)K + (size_t)j) * 64;
+                const int NS = g0 == 0 ? ring_table(g, G, g.irmin, n, S, lane, -1, tab_f + row, tab_i + row)
+                                       : ring_table(g, G, g.irmin + g0, n, S, lane, -1);
                 T += lane < n ? S.ri[lane].y : 0;
                 nacc += walk_pv<0, ACC, false>(g.pv, S, NS, lane, g.K4, 0.0f, map);
                 wave_sync();
@@ -335,6 +391,8 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
                 }
             }
         }
+        // the cached ring terms are read back below
+        __builtin_amdgcn_s_waitcnt(0);
         // ---- the group's window
         const int R0 = wave_min_i(my_on ? my_lo : 0x7fffffff);
         const int R1 = wave_max_i(my_on ? my_hi : -0x7fffffff);
@@ -392,24 +450,61 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
             }
             wave_sync();
             const int b1 = b0 + BH - 1;
-            for (int j = 0; j < nm; ++j) {
+            // members meeting the band, in order; the cached ring terms of
+            // the next one are loaded while the current one is walked
+            const size_t wrow = (size_t)blockIdx.x * (size_t)K;
+            auto next_member = [&](int j) {
+                for (++j; j < nm; ++j) {
+                    const int lo = wave_read(my_lo, j), hi = wave_read(my_hi, j);
+                    if (wave_read((int)my_on, j) && hi >= b0 && lo <= b1) break;
+                }
+                return j;
+            };
+            auto fetch = [&](int j, float4 &f, int4 &i) {
                 const int lo = wave_read(my_lo, j), hi = wave_read(my_hi, j);
-                if (!wave_read((int)my_on, j) || hi < b0 || lo > b1) continue;
-                const float sc = wave_read(my_scale, j);
-                // the member's ring-walk terms, from its lane (computed in phase 1)
-                PGeo g;
-                g.phi = wave_read(my_phi, j);
-                g.z0 = wave_read(my_z0, j);
-                g.s0 = wave_read(my_s0, j);
-                g.xa = wave_read(my_xa, j);
-                g.cosr = wave_read(my_cosr, j);
-                g.K4 = wave_read(my_K4, j);
                 const int rlo = b0 > lo ? b0 : lo, rhi = b1 < hi ? b1 : hi;
-                const int NS = ring_table(g, G, rlo, rhi - rlo + 1, S, lane, b0);
+                const int idx = rlo - lo + lane;
+                if (lane <= rhi - rlo && idx < 64) {
+                    f = tab_f[(wrow + (size_t)j) * 64 + (size_t)idx];
+                    i = tab_i[(wrow + (size_t)j) * 64 + (size_t)idx];
+                }
+            };
+            int j = next_member(-1);
+            float4 nf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            int4 ni = make_int4(0, 0, 0, 0);
+            if (j < nm) fetch(j, nf, ni);
+            while (j < nm) {
+                const float4 cf = nf;
+                const int4 ci = ni;
+                const int jn = next_member(j);
+                if (jn < nm) fetch(jn, nf, ni);
+                const int lo = wave_read(my_lo, j), hi = wave_read(my_hi, j);
+                const int rlo = b0 > lo ? b0 : lo, rhi = b1 < hi ? b1 : hi;
+                const bool on = lane <= rhi - rlo;
+                RingLite rl;
+                rl.cnt = 0;
+                if (on) {
+                    if (rlo - lo + lane < 64) {
+                        rl = ring_unpack(cf, ci);
+                    } else {
+                        // rings past the cached 64 of a tall member
+                        PGeo g;
+                        g.phi = wave_read(my_phi, j);
+                        g.z0 = wave_read(my_z0, j);
+                        g.s0 = wave_read(my_s0, j);
+                        g.xa = wave_read(my_xa, j);
+                        g.cosr = wave_read(my_cosr, j);
+                        rl = ring_lite(g, G, rlo + lane);
+                    }
+                }
+                const int NS = ring_table_fill(rl, on, rlo, S, lane, b0);
+                const float sc = wave_read(my_scale, j);
+                const float K4 = wave_read(my_K4, j);
                 const int pv = wave_read(my_pv, j);
-                if (direct) walk_pv<1, ACC, false>(pv, S, NS, lane, g.K4, sc, map);
-                else walk_pv<1, ACC, true>(pv, S, NS, lane, g.K4, sc, map);
+                if (direct) walk_pv<1, ACC, false>(pv, S, NS, lane, K4, sc, map);
+                else walk_pv<1, ACC, true>(pv, S, NS, lane, K4, sc, map);
                 wave_sync();
+                j = jn;
             }
             if (!direct) {
                 flush_band<ACC>(S, Sb, lane, map, npix);
@@ -526,9 +621,11 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
     k_sort_scatter<<<gc.sort_blocks, 256, 0, st>>>(gc.keys, counts, gc.hist, gc.idx);
     if (sorted) (void)hipEventRecord(sorted, st);
     if (acc == kAccComp)
-        k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work);
+        k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work,
+                                                               gc.tab_f, gc.tab_i);
     else
-        k_disc_group<kAccF32><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work);
+        k_disc_group<kAccF32><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work,
+                                                              gc.tab_f, gc.tab_i);
     return hipGetLastError();
 }
 

@@ -78,6 +78,10 @@ struct GroupCfg {
     unsigned *hist;   // 12 nside_c^2
     unsigned *sums;   // ceil(12 nside_c^2 / 1024)
     unsigned *work;   // group counter
+    // per-wave cache of the members' ring terms (group_blocks x k x 64
+    // rings), written by the normalisation pass, read back per band
+    float4 *tab_f;
+    int4 *tab_i;
 };
 
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, const GroupCfg &gc, MapRef map,
