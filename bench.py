@@ -36,6 +36,14 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 SLOTS_PER_EVAL = 16
 EVALS_PER_CONTRIB = 2
 
+# Memory traffic of one launch of the grouped kernel, from two rocprofv3
+# counter passes (FETCH_SIZE, WRITE_SIZE in KiB) scaled by what the same
+# counters read for k_finalize, a streaming kernel of known size (fetch x2,
+# write x1): profiles/k_disc_group_nfw_sub2000_near_1p1e9_nside2048_traffic.md
+MEASURED_TRAFFIC = {
+    ("nfw_sub2000_near_1p1e9", 2048, "compensated"): (1.15849e9 * 2.0 + 8.82174e9 * 1.0) * 1024,
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -151,11 +159,15 @@ def total_flux_check(torch, parts, params, m, nside):
     return abs(got - float(tot)) / float(tot)
 
 
-def cpu_baseline(pw, args, nside):
+def cpu_baseline(pw, args, nside, n_full, c_full):
     """The oracle (OpenMP, all host cores given to the job) on a bounded
-    sample of the same workload."""
+    sample of the same workload, sized to about --cpu-seconds; its
+    footprint-pixel rate gives the time of the whole workload (work is
+    proportional to the footprint pixels), reported as Mparticles/s.  The
+    sample takes the near-observer particles only pro rata (none in a small
+    sample): one sky-covering particle would otherwise hold a thread for the
+    whole run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle
     oracle.build()
     cores = min(16, os.cpu_count() or 1)
@@ -164,30 +176,38 @@ def cpu_baseline(pw, args, nside):
     pw.write_ini(ini, nside=nside)
     cfg = oracle.config_from_params(oracle.params_from_ini(ini))
     os.unlink(ini)
-    # calibrate on a small sample, then size the timed sample to ~cpu_seconds
-    m = 2000
-    recs = pw.synth.to_records(hm.sample(m))
-    t0 = time.perf_counter()
-    oracle.genmap(recs, cfg, nthreads=cores, finalize=False)
-    dt = time.perf_counter() - t0
-    m2 = int(max(2000, min(hm.n, m * args.cpu_seconds / max(dt, 1e-3))))
-    recs = pw.synth.to_records(hm.sample(m2))
-    t0 = time.perf_counter()
-    _, st = oracle.genmap(recs, cfg, nthreads=cores, finalize=False)
-    dt = time.perf_counter() - t0
-    # the reference's generator is single-threaded: time one core on a
-    # quarter of the sample as well
-    m1 = max(500, m2 // (4 * cores))
-    recs1 = pw.synth.to_records(hm.sample(m1))
-    t0 = time.perf_counter()
-    _, st1 = oracle.genmap(recs1, cfg, nthreads=1, finalize=False)
-    dt1 = time.perf_counter() - t0
-    return {"value": m2 / dt / 1e6, "unit": "Mparticles/s", "cores": cores, "kind": "port",
-            "sample": f"{m2} particles of the same model (16 index windows + near-observer share), "
-                      f"nside {nside}, {st['contributions']} footprint pixels, {dt:.1f} s with "
-                      f"{cores} OpenMP threads",
-            "single_core": {"value": m1 / dt1 / 1e6, "sample": f"{m1} particles, {st1['contributions']} "
-                            f"footprint pixels, {dt1:.1f} s on 1 thread"}}
+
+    def timed(m, threads):
+        recs = pw.synth.to_records(hm.sample(m, near_min=0))
+        t0 = time.perf_counter()
+        _, st = oracle.genmap(recs, cfg, nthreads=threads, finalize=False)
+        return time.perf_counter() - t0, st["contributions"]
+
+    def sized(threads, seconds):
+        # marginal rate between a small and a large sample, so that the
+        # per-thread map set-up and reduction (fixed costs) cancel; the
+        # large sample grows until its extra time reaches half the target
+        dt_a, c_a = timed(1000, threads)
+        m = 4000
+        for _ in range(8):
+            dt, c = timed(m, threads)
+            if dt - dt_a >= 0.5 * seconds or m >= hm.n:
+                break
+            m = int(min(hm.n, m * min(8.0, seconds / max(dt - dt_a, 1e-2))))
+        return m, dt, c, (c - c_a) / max(dt - dt_a, 1e-9)
+
+    m, dt, c, rate = sized(cores, args.cpu_seconds)
+    m1, dt1, c1, rate1 = sized(1, args.cpu_seconds / 3)
+    return {"value": n_full / (c_full / rate) / 1e6, "unit": "Mparticles/s", "cores": cores, "kind": "port",
+            "sample": f"{m} particles of the same model (16 index windows, near-observer share pro rata), "
+                      f"nside {nside}: "
+                      f"{c} footprint pixels in {dt:.1f} s on {cores} OpenMP threads, marginal rate over a "
+                      f"1000-particle sample {rate:.3e} pixels/s, "
+                      f"extrapolated to the workload's {c_full} footprint pixels ({c_full / rate:.0f} s)",
+            "sample_value": m / dt / 1e6,
+            "single_core": {"value": n_full / (c_full / rate1) / 1e6,
+                            "sample": f"{m1} particles, {c1} footprint pixels in {dt1:.1f} s on 1 thread, "
+                                      f"marginal rate {rate1:.3e} pixels/s"}}
 
 
 def main():
@@ -260,7 +280,7 @@ def main():
         "peak": VALU_PEAK_TOPS,
         "unit": "Tlane-op/s",
         "frac": achieved_valu / VALU_PEAK_TOPS,
-        "traffic": None,
+        "traffic": MEASURED_TRAFFIC.get((args.workload, nside, args.precision)) if args.n is None else None,
         "kernel": "k_disc_group (medium footprint kernel)",
         "kernel_ms": kms,
         "contributions": {"medium": C_med, "large": C_large},
@@ -296,7 +316,7 @@ def main():
     if not args.no_cpu:
         del parts
         torch.cuda.empty_cache()
-        line["cpu_baseline"] = cpu_baseline(pw, args, nside)
+        line["cpu_baseline"] = cpu_baseline(pw, args, nside, n, C)
     print(json.dumps(line), flush=True)
 
 

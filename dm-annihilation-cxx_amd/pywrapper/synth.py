@@ -153,14 +153,15 @@ class HaloModel:
         out[:, 3:6] = pos.to(torch.float32)
         return out
 
-    def sample(self, m: int, windows: int = 16) -> torch.Tensor:
+    def sample(self, m: int, windows: int = 16, near_min: int = 1) -> torch.Tensor:
         """m particles of the full model (same densities and smoothing
         lengths as the full set): `windows` evenly spaced contiguous index
-        windows, plus the near-observer particles pro rata."""
+        windows, plus the near-observer particles pro rata (at least
+        near_min of them)."""
         m = int(min(m, self.n))
         near = 0
         if self.n_near:
-            near = max(1, int(round(m * self.n_near / self.n)))
+            near = max(near_min, int(round(m * self.n_near / self.n)))
         body = m - near
         per = max(1, body // windows)
         n_body = self.n - self.n_near

@@ -673,7 +673,10 @@ void orc_genmap_omp(const orc_particle *parts, int64_t n, const orc_config *c, d
     {
         int t = omp_get_thread_num();
         maps[t] = (double *)calloc((size_t)npix, sizeof(double));
-        int64_t chunk = 4096;
+        // small tasks keep the threads balanced on heavy-tailed footprints
+        int64_t chunk = n / ((int64_t)nthreads * 64);
+        if (chunk < 16) chunk = 16;
+        if (chunk > 4096) chunk = 4096;
 #pragma omp for schedule(dynamic, 1)
         for (int64_t b = 0; b < (n + chunk - 1) / chunk; b++) {
             int64_t lo = b * chunk, hi = lo + chunk < n ? lo + chunk : n;
