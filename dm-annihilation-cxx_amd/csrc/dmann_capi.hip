@@ -95,7 +95,7 @@ static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
         return hipSuccess;
     }
     const int64_t cap = c->max_parts ? c->max_parts : 1;
-    const int64_t nbins = 12LL * nside_c * nside_c;
+    const int64_t nbins = 12LL * nside_c * nside_c * kSizeBuckets;
     hipError_t e;
     if ((e = hipMalloc(&c->gc.keys, cap * sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMalloc(&c->gc.idx, cap * sizeof(unsigned))) != hipSuccess) return e;
@@ -220,7 +220,8 @@ int dmann_set_grouping(dmann_ctx *c, int k, int nside_c)
 {
     if (!c) return DMANN_ERR_ARG;
     if (k < 0 || k > 64) return fail(c, DMANN_ERR_ARG, "group size %d outside 0..64", k);
-    if (nside_c < 1 || nside_c > c->P.nside) return fail(c, DMANN_ERR_ARG, "sort cell nside %d outside 1..nside", nside_c);
+    if (nside_c < 1 || nside_c > c->P.nside || nside_c > kMaxSortNside)
+        return fail(c, DMANN_ERR_ARG, "sort cell nside %d outside 1..min(nside, %d)", nside_c, kMaxSortNside);
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, alloc_group(c, k, nside_c));

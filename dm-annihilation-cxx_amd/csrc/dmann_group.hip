@@ -522,15 +522,18 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
 
 // ---------------------------------------------------------------- sorting
 
-// Coarse sky cell (RING pixel at nside_c) of every medium-class record and
-// its histogram.
+// Sort key of every medium-class record -- coarse sky cell (RING pixel at
+// nside_c) times kSizeBuckets plus the footprint-size class -- and the keys'
+// histogram.
 __global__ void k_sort_keys(const DiscRec *__restrict__ recs, const unsigned *__restrict__ counts, int nside_c,
                             unsigned *__restrict__ keys, unsigned *__restrict__ hist)
 {
     const unsigned n = counts[0];
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const DiscRec r = recs[i];
-        const unsigned key = (unsigned)ang2pix(nside_c, r.theta, r.phi);
+        const int lg = r.est > 0 ? 31 - __clz(r.est) : 0;
+        const int bucket = lg < 6 ? 0 : (lg - 6 > kSizeBuckets - 1 ? kSizeBuckets - 1 : lg - 6);
+        const unsigned key = (unsigned)ang2pix(nside_c, r.theta, r.phi) * (unsigned)kSizeBuckets + (unsigned)bucket;
         keys[i] = key;
         atomicAdd(&hist[key], 1u);
     }
@@ -611,7 +614,7 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
                         unsigned long long *stats, hipStream_t st, hipEvent_t sorted)
 {
     k_zero_u32<<<1, 64, 0, st>>>(gc.work, 1);
-    const int nbins = 12 * gc.nside_c * gc.nside_c;
+    const int nbins = 12 * gc.nside_c * gc.nside_c * kSizeBuckets;
     const int nb = (nbins + 1023) / 1024;
     k_zero_u32<<<nb < 4096 ? nb : 4096, 256, 0, st>>>(gc.hist, nbins);
     k_sort_keys<<<gc.sort_blocks, 256, 0, st>>>(recs, counts, gc.nside_c, gc.keys, gc.hist);

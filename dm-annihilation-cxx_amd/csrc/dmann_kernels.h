@@ -68,6 +68,12 @@ struct LaunchCfg {
 
 // Grouped rendering of the medium class (dmann_group.hip): counting sort by
 // coarse sky cell, then K neighbours per wavefront with LDS accumulation.
+// Sort key of a medium record: coarse sky cell, then footprint-size class
+// (floor(log2 est) - 6, clamped to 0..15), so that a group of neighbours
+// also holds footprints of similar size and its window stays tight.
+constexpr int kSizeBuckets = 16;
+constexpr int kMaxSortNside = 1024;
+
 struct GroupCfg {
     int k;            // particles per group (0: per-particle kernel instead)
     int nside_c;      // sort cell resolution

@@ -123,10 +123,11 @@ int dmann_render_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t
 int dmann_set_tuning(dmann_ctx *ctx, int medium_blocks_per_cu, int large_blocks_per_cu, int large_pix);
 
 /* Grouped rendering of the medium class (no counterpart in the reference):
- * particles are sorted by RING cell at nside_c (1..nside) and K (1..64)
- * neighbours share one wavefront and an LDS window, cutting global atomics
- * ~K-fold; k = 0 renders one particle per wavefront with per-pixel atomics.
- * Defaults K = 16, nside_c = nside / 8.  Ignored by the FP64 form. */
+ * particles are sorted by RING cell at nside_c (1..min(nside, 1024)), then
+ * by footprint-size class, and K (1..64) neighbours share one wavefront and
+ * an LDS window, cutting global atomics ~K-fold; k = 0 renders one particle
+ * per wavefront with per-pixel atomics.  Defaults K = 16, nside_c =
+ * nside / 8.  Ignored by the FP64 form. */
 int dmann_set_grouping(dmann_ctx *ctx, int k, int nside_c);
 
 /* Uses `stream` (a hipStream_t, NULL = the null stream) for later calls. */
