@@ -118,6 +118,8 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
             return bfail(nullptr, DMANN_ERR_ARG, "map %d: nside %d outside 1..%d", m, params[m].nside, kMaxNside);
         if (params[m].precision != prec)
             return bfail(nullptr, DMANN_ERR_ARG, "map %d: all maps of a batch share one precision", m);
+        if (params[m].flux_kind < DMANN_FLUX_ANN_NOSOM || params[m].flux_kind > DMANN_FLUX_ANN_ADIABATIC)
+            return bfail(nullptr, DMANN_ERR_ARG, "map %d: unknown flux plug-in %d", m, params[m].flux_kind);
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(nullptr, DMANN_ERR_HIP, "no HIP device");
@@ -142,16 +144,8 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
     for (int m = 0; m < nmaps; m++) {
         const dmann_params &p = params[m];
-        DevParams P{};
-        for (int i = 0; i < 3; i++) P.opos[i] = p.opos[i];
-        for (int i = 0; i < 9; i++) P.rot[i] = p.rotmatrix[i];
-        P.flux_factor = p.flux_factor;
-        P.mass_cut = p.mass_cut;
-        P.nside = p.nside;
-        P.npix = (int)npix_of(p.nside);
-        P.dOmega = 4 * kPi / (double)P.npix;
-        P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592));
-        P.large_pix = 1 << 16;
+        DevParams P;
+        fill_devparams(&p, P, 1 << 16);
         b->P.push_back(P);
         b->npix.push_back(P.npix);
         MapRef r{nullptr, nullptr, nullptr};

@@ -127,6 +127,27 @@ static void free_all(dmann_ctx *c)
     c->d_out = nullptr;
 }
 
+namespace dmann {
+void fill_devparams(const dmann_params *p, DevParams &P, int large_pix)
+{
+    P = DevParams{};
+    for (int i = 0; i < 3; i++) P.opos[i] = p->opos[i];
+    for (int i = 0; i < 9; i++) P.rot[i] = p->rotmatrix[i];
+    P.flux_factor = p->flux_factor;
+    P.mass_cut = p->mass_cut;
+    P.nside = p->nside;
+    P.npix = (int)npix_of(p->nside);
+    P.dOmega = 4 * kPi / (double)P.npix;              // Map::setNside, datatypes.h:248-253
+    P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592)); // src/genmap.cpp:38
+    P.large_pix = large_pix;
+    P.flux_kind = p->flux_kind;
+    P.decay_factor = p->decay_factor;
+    P.vel_to_cgs = p->vel_to_cgs;
+    P.c_in_cgs = p->c_in_cgs;
+    for (int i = 0; i < 3; i++) P.cpos[i] = p->cpos[i];
+}
+} // namespace dmann
+
 extern "C" {
 
 int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int device)
@@ -135,6 +156,8 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     *out = nullptr;
     if (p->nside < 1 || p->nside > kMaxNside)
         return fail(nullptr, DMANN_ERR_ARG, "nside %d outside 1..%d", p->nside, kMaxNside);
+    if (p->flux_kind < DMANN_FLUX_ANN_NOSOM || p->flux_kind > DMANN_FLUX_ANN_ADIABATIC)
+        return fail(nullptr, DMANN_ERR_ARG, "unknown flux plug-in %d", p->flux_kind);
     if (max_parts < 0 || max_parts > (int64_t)INT32_MAX * 2)
         return fail(nullptr, DMANN_ERR_ARG, "max_parts %lld outside 0..%lld", (long long)max_parts,
                     (long long)INT32_MAX * 2);
@@ -151,16 +174,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     c->acc = p->precision == DMANN_PREC_FP32 ? kAccF32 : p->precision == DMANN_PREC_COMPENSATED ? kAccComp : kAccF64;
     c->max_parts = max_parts;
     c->npix = npix_of(p->nside);
-    DevParams &P = c->P;
-    for (int i = 0; i < 3; i++) P.opos[i] = p->opos[i];
-    for (int i = 0; i < 9; i++) P.rot[i] = p->rotmatrix[i];
-    P.flux_factor = p->flux_factor;
-    P.mass_cut = p->mass_cut;
-    P.nside = p->nside;
-    P.npix = (int)c->npix;
-    P.dOmega = 4 * kPi / (double)c->npix;              // Map::setNside, datatypes.h:248-253
-    P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592)); // src/genmap.cpp:38
-    P.large_pix = 1 << 22;
+    fill_devparams(p, c->P, 1 << 22);
 
     hipError_t e = hipSetDevice(device);
     int dev_cus = 256;
@@ -266,6 +280,8 @@ int dmann_adjoint_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t 
     if (n < 0) return fail(c, DMANN_ERR_ARG, "negative particle count");
     if (n == 0) return DMANN_OK;
     if (!dev_parts || !dev_grad_map || !dev_grad_parts) return fail(c, DMANN_ERR_ARG, "null pointer");
+    if (c->P.flux_kind == DMANN_FLUX_ANN_ADIABATIC)
+        return fail(c, DMANN_ERR_ARG, "no adjoint for the adiabatic flux plug-in (its factor depends on position)");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, launch_adjoint(reinterpret_cast<const Particle *>(dev_parts), n, c->P, dev_grad_map, dev_grad_parts,
                              c->cfg.medium_blocks, c->stream));
@@ -385,18 +401,7 @@ void dmann_destroy(dmann_ctx *c)
 
 const char *dmann_last_error(const dmann_ctx *c) { return c ? c->msg : g_create_msg; }
 
-static void fill_devparams(const dmann_params *p, DevParams &P)
-{
-    for (int i = 0; i < 3; i++) P.opos[i] = p->opos[i];
-    for (int i = 0; i < 9; i++) P.rot[i] = p->rotmatrix[i];
-    P.flux_factor = p->flux_factor;
-    P.mass_cut = p->mass_cut;
-    P.nside = p->nside;
-    P.npix = (int)npix_of(p->nside);
-    P.dOmega = 4 * kPi / (double)P.npix;
-    P.theta0 = acos(1.0 - P.dOmega / (2.0 * 3.141592));
-    P.large_pix = 1 << 22;
-}
+static void fill_devparams(const dmann_params *p, DevParams &P) { dmann::fill_devparams(p, P, 1 << 22); }
 
 int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)
 {
@@ -428,7 +433,9 @@ int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)
 int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int32_t *pix, float *w, int64_t cap,
                              double *flux, int *kind)
 {
-    if (!p || !q || p->nside < 1 || p->nside > kMaxNside) return -1;
+    if (!p || !q || p->nside < 1 || p->nside > kMaxNside || p->flux_kind < DMANN_FLUX_ANN_NOSOM ||
+        p->flux_kind > DMANN_FLUX_ANN_ADIABATIC)
+        return -1;
     DevParams P;
     fill_devparams(p, P);
     DiscRec rec;

@@ -17,8 +17,56 @@ namespace dmann {
 // both poles.  s0 = sin(theta) as sqrt((1 - cos theta)(1 + cos theta)).
 DMANN_HD bool fast_geometry(double rad, double s0) { return rad <= 0.5 && sin(rad) < s0 * 0.43; }
 
+// getflux of the map's flux plug-in for particle q at distance d, with the
+// reference's operation order (src/flux/flux_ann_nosom.cpp:12-23,
+// flux_ann_pwave.cpp:13-29, flux_ann_somv.cpp:16-43, flux_ann_somv2.cpp:16-44,
+// flux_decay.cpp:10-16, flux_ann_adiabatic.cpp:10-31).
+DMANN_HD double plugin_flux(const DevParams &P, const Particle &q, double d)
+{
+    const double K = P.flux_factor;
+    switch (P.flux_kind) {
+    case DMANN_FLUX_ANN_PWAVE: {
+        const double v = q.sigmav;
+        double somer = (v * P.vel_to_cgs) / P.c_in_cgs;
+        somer = somer * somer;
+        return K * somer * (double)q.dens * (double)q.mass / (4.0 * kPi * d * d);
+    }
+    case DMANN_FLUX_ANN_SOMV: {
+        double v = (double)q.sigmav * P.vel_to_cgs;
+        v /= 1.0e5; // km/s, saturated at 1 km/s
+        if (v < 1) v = 1;
+        const double somer = P.c_in_cgs / v / 1.0e4;
+        return K * somer * (double)q.dens * (double)q.mass / (4.0 * kPi * d * d);
+    }
+    case DMANN_FLUX_ANN_SOMV2: {
+        double v = (double)q.sigmav * P.vel_to_cgs;
+        v /= 1.0e5; // km/s, saturated at 5 km/s
+        if (v < 5) v = 5;
+        double somer = P.c_in_cgs / v;
+        somer *= (somer / 1.0e8);
+        return K * somer * (double)q.dens * (double)q.mass / (4.0 * kPi * d * d);
+    }
+    case DMANN_FLUX_DECAY:
+        return P.decay_factor * (double)q.mass / (4.0 * kPi * d * d);
+    case DMANN_FLUX_ANN_ADIABATIC: {
+        // profile factor of the distance to the halo centre, in kpc/h
+        const double dx = (double)q.posx - P.cpos[0];
+        const double dy = (double)q.posy - P.cpos[1];
+        const double dz = (double)q.posz - P.cpos[2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz) * 40000.0;
+        const double factor =
+            (14.6534 * pow(1 + 0.0355872 * r, 1.76)) / (pow(1 + 0.10888 * pow(r, 0.76), 3.34211) * pow(r, 0.13));
+        double f = K * (double)q.dens * (double)q.mass / (4.0 * kPi * d * d);
+        f *= (factor * factor);
+        return f;
+    }
+    default:
+        return K * (double)q.dens * (double)q.mass / (4.0 * kPi * d * d);
+    }
+}
+
 // Pre-pass of one particle (src/genmap.cpp:82-110; src/anglefuncs.cpp:15-71;
-// src/flux/flux_ann_nosom.cpp:12-23).  Returns kClsSkip (filtered),
+// the flux plug-in, plugin_flux).  Returns kClsSkip (filtered),
 // kClsPoint (whole flux to rec.p0), kClsMedium or kClsLarge (footprint).
 // *bad is set when the direction is undefined (the reference exits there).
 DMANN_HD int prepass_one(const DevParams &P, const Particle &q, DiscRec &rec, bool &bad)
@@ -29,7 +77,7 @@ DMANN_HD int prepass_one(const DevParams &P, const Particle &q, DiscRec &rec, bo
     const double dy = (double)q.posy - P.opos[1];
     const double dz = (double)q.posz - P.opos[2];
     const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    const double flux = P.flux_factor * (double)q.dens * (double)q.mass / (4.0 * kPi * d * d);
+    const double flux = plugin_flux(P, q, d);
     const double v[3] = {dx, dy, dz};
     double t[3];
     for (int a = 0; a < 3; a++) {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dmann.h"
+
 namespace dmann {
 
 // 48-byte particle record (src/datatypes.h:46-105).
@@ -50,8 +52,13 @@ struct DevParams {
     int nside;
     int npix;
     int large_pix;  // estimated footprint above which the sky-covering path is used
-    int pad;
+    int flux_kind;  // dmann_flux
+    double decay_factor, vel_to_cgs, c_in_cgs;
+    double cpos[3];
 };
+
+// Device constants of one map from the C-ABI parameters.
+void fill_devparams(const dmann_params *p, DevParams &P, int large_pix);
 
 // Accumulation targets of one map.
 struct MapRef {

@@ -463,9 +463,14 @@ __global__ __launch_bounds__(256) void k_adjoint(const Particle *__restrict__ pa
                 dLdr = flux * (Sr * N - S * Nr) / (N * N);
                 for (int e = 0; e < 3; e++) dLdv[e] = flux * (Sv[e] * N - S * Nv[e]) / (N * N);
             }
-            const double inv4pid2 = 1.0 / (4.0 * kPi * d * d);
-            res[0] = dLdflux * P.flux_factor * (double)q.dens * inv4pid2;
-            res[1] = dLdflux * P.flux_factor * (double)q.mass * inv4pid2;
+            // every plug-in is linear in the mass, the annihilation ones in
+            // the density too (the decay one does not depend on it)
+            Particle q1 = q;
+            q1.mass = 1.0f;
+            res[0] = dLdflux * plugin_flux(P, q1, d);
+            q1 = q;
+            q1.dens = 1.0f;
+            res[1] = P.flux_kind == DMANN_FLUX_DECAY ? 0.0 : dLdflux * plugin_flux(P, q1, d);
             res[2] = dLdr / d;
             const double dLdd = dLdflux * (-2.0 * flux / d) + dLdr * (-h / (d * d));
             // R D dL/dv, D = diag(-1, -1, 1); R row-major

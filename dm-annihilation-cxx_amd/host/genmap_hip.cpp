@@ -9,7 +9,10 @@
 //
 // Accumulation form: DMANN_PRECISION=fp32 | compensated | fp64 in the
 // environment (default compensated, which meets the 1e-5 per-pixel bound
-// against the FP64 CPU generator).
+// against the FP64 CPU generator).  Flux plug-in: DMANN_FLUX=ann_nosom
+// (default) | ann_pwave | ann_somv | ann_somv2 | decay | ann_adiabatic -- the
+// reference links one src/flux/flux_<kind>.cpp per executable
+// (annmap_cuda_nosom, annmap_cuda_pwave, ..., src/Makefile:130-160).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,11 +30,23 @@ static int precision_from_env()
     return DMANN_PREC_COMPENSATED;
 }
 
+static int flux_from_env()
+{
+    static const char *names[] = {"ann_nosom", "ann_pwave", "ann_somv", "ann_somv2", "decay", "ann_adiabatic"};
+    const char *s = getenv("DMANN_FLUX");
+    if (!s || !*s) return DMANN_FLUX_ANN_NOSOM;
+    for (int k = 0; k < 6; k++)
+        if (!strcmp(s, names[k])) return k;
+    printf("Unknown flux plug-in %s\n", s);
+    exit(1);
+}
+
 void MapGenerator::start()
 {
     const int Npix = par_->map.Npix;
     dmann_params p;
     par_->toDmannParams(&p, precision_from_env());
+    p.flux_kind = flux_from_env();
     int64_t cap = par_->memParts > 0 ? par_->memParts : 1;
     dmann_ctx *ctx = nullptr;
     if (dmann_create(&ctx, &p, cap, 0) != DMANN_OK) {

@@ -274,6 +274,15 @@ void Parameters::toDmannParams(dmann_params *p, int precision) const
     p->mass_cut = massCut();
     p->nside = map.Nside;
     p->precision = precision;
+    p->flux_kind = DMANN_FLUX_ANN_NOSOM;
+    p->reserved = 0;
+    // unit factor of src/flux/flux_decay.cpp:11-14
+    const double decay_flux_to_cgs = (codeunits.mass_to_cgs / pow(codeunits.length_to_cgs, 2));
+    p->decay_factor =
+        pow((natconst.c_in_cgs), 2) / (units.eV_in_cgs * 1.0e9) / (units.Mpc_in_cgs * 1.0e-3) * decay_flux_to_cgs;
+    p->vel_to_cgs = codeunits.velocity_to_cgs;
+    p->c_in_cgs = natconst.c_in_cgs;
+    for (int i = 0; i < 3; i++) p->cpos[i] = params.cpos[i];
 }
 
 static std::string join(const double *d, int n)

@@ -27,10 +27,23 @@ PARTICLE_DTYPE = np.dtype([(n, "<f4") for n in PARTICLE_FIELDS])
 FLT_MAX = float(np.finfo(np.float32).max)
 
 
+# flux plug-ins (the reference's src/flux/flux_<kind>.cpp), DmannParams.flux_kind
+FLUX_KINDS = ("ann_nosom", "ann_pwave", "ann_somv", "ann_somv2", "decay", "ann_adiabatic")
+
+
 class DmannParams(C.Structure):
     _fields_ = [("opos", C.c_double * 3), ("rotmatrix", C.c_double * 9),
                 ("flux_factor", C.c_double), ("mass_cut", C.c_double),
-                ("nside", C.c_int32), ("precision", C.c_int32)]
+                ("nside", C.c_int32), ("precision", C.c_int32),
+                ("flux_kind", C.c_int32), ("reserved", C.c_int32),
+                ("decay_factor", C.c_double), ("vel_to_cgs", C.c_double), ("c_in_cgs", C.c_double),
+                ("cpos", C.c_double * 3)]
+
+    def with_flux(self, kind: str) -> "DmannParams":
+        """A copy that uses the flux plug-in `kind` (one of FLUX_KINDS)."""
+        p = self.copy()
+        p.flux_kind = FLUX_KINDS.index(kind)
+        return p
 
     def copy(self) -> "DmannParams":
         p = DmannParams()

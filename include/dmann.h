@@ -69,7 +69,14 @@ enum dmann_status {
  *             mass >= mass_cut or dens < 0 are skipped (src/genmap.cpp:52-58,82-85).
  *             Use FLT_MAX*1.1 when no masses are configured.
  *  nside      HEALPix nside, 1..8192 (src/chealpixrender.cu:25 NS_MAX)
- *  precision  enum dmann_precision                                           */
+ *  precision  enum dmann_precision
+ *  flux_kind  enum dmann_flux: the flux plug-in (src/flux/flux_*.cpp, one of
+ *             which the reference links into each executable,
+ *             src/Makefile:130-160); the terms below feed the plug-ins:
+ *  decay_factor  unit factor of src/flux/flux_decay.cpp:11-14
+ *  vel_to_cgs    codeunits.velocity_to_cgs (pwave, somv, somv2)
+ *  c_in_cgs      natconst.c_in_cgs
+ *  cpos          params.cpos, the halo centre (adiabatic)                    */
 typedef struct dmann_params {
     double opos[3];
     double rotmatrix[9];
@@ -77,7 +84,30 @@ typedef struct dmann_params {
     double mass_cut;
     int32_t nside;
     int32_t precision;
+    int32_t flux_kind;
+    int32_t reserved;
+    double decay_factor;
+    double vel_to_cgs;
+    double c_in_cgs;
+    double cpos[3];
 } dmann_params;
+
+/* Flux plug-ins: K rho m / (4 pi d^2) times
+ *   ANN_NOSOM      1                                  (flux_ann_nosom.cpp)
+ *   ANN_PWAVE      (sigmav v_unit / c)^2              (flux_ann_pwave.cpp)
+ *   ANN_SOMV       c / max(v, 1 km/s) / 1e4           (flux_ann_somv.cpp)
+ *   ANN_SOMV2      (c / max(v, 5 km/s))^2 / 1e8       (flux_ann_somv2.cpp)
+ *   ANN_ADIABATIC  the adiabatic-contraction profile factor squared, of the
+ *                  distance to the halo centre        (flux_ann_adiabatic.cpp)
+ * and DECAY = decay_factor m / (4 pi d^2)             (flux_decay.cpp). */
+enum dmann_flux {
+    DMANN_FLUX_ANN_NOSOM = 0,
+    DMANN_FLUX_ANN_PWAVE = 1,
+    DMANN_FLUX_ANN_SOMV = 2,
+    DMANN_FLUX_ANN_SOMV2 = 3,
+    DMANN_FLUX_DECAY = 4,
+    DMANN_FLUX_ANN_ADIABATIC = 5
+};
 
 /* Counters of the last dmann_get_stats call (cumulative since create/reset). */
 typedef struct dmann_stats {
@@ -144,7 +174,8 @@ int dmann_synchronize(dmann_ctx *ctx);
  * posx, posy, posz) of particle i (FP64, device).  The footprint is recomputed
  * with the forward's ring walk; weights and derivatives are FP64.  Pixel
  * membership of the footprint is piecewise constant: its jumps are not part of
- * the gradient.  Asynchronous on the context's stream. */
+ * the gradient.  Every flux plug-in except ANN_ADIABATIC (DMANN_ERR_ARG).
+ * Asynchronous on the context's stream. */
 int dmann_adjoint_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t n, const double *dev_grad_map,
                          double *dev_grad_parts);
 

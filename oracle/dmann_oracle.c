@@ -467,6 +467,13 @@ void orc_config_from_params(const orc_params *p, orc_config *c)
         if (p->particle_masses[i] > 0 && p->particle_masses[i] < hires)
             hires = (float)p->particle_masses[i];
     c->mass_cut = (double)hires * 1.1;
+    c->flux_kind = ORC_FLUX_ANN_NOSOM;
+    /* flux_decay.cpp:11-14 */
+    double decay_flux_to_cgs = (p->mass_to_cgs / pow(p->length_to_cgs, 2));
+    c->decay_factor = pow(kc_in_cgs, 2) / (keV_in_cgs * 1.0e9) / (kMpc_in_cgs * 1.0e-3) * decay_flux_to_cgs;
+    c->vel_to_cgs = p->velocity_to_cgs;
+    c->c_in_cgs = kc_in_cgs;
+    for (int i = 0; i < 3; i++) c->cpos[i] = p->cpos[i];
 }
 
 /* ===================================================================== */
@@ -477,6 +484,55 @@ double orc_getflux(double flux_factor, const orc_particle *q, double d)
 {
     /* flux_ann_nosom.cpp:18 */
     return flux_factor * (double)q->dens * (double)q->mass / (4.0 * ORC_PI * d * d);
+}
+
+double orc_getflux_cfg(const orc_config *c, const orc_particle *q, double d)
+{
+    const double K = c->flux_factor;
+    switch (c->flux_kind) {
+    case ORC_FLUX_ANN_PWAVE: {
+        /* flux_ann_pwave.cpp:18-27: (v/c)^2 */
+        double v = q->sigmav;
+        double somer_v = (v * c->vel_to_cgs) / c->c_in_cgs;
+        somer_v = somer_v * somer_v;
+        return K * somer_v * q->dens * q->mass / (4.0 * ORC_PI * d * d);
+    }
+    case ORC_FLUX_ANN_SOMV: {
+        /* flux_ann_somv.cpp:21-40: c / v / 1e4, v in km/s saturated at 1 */
+        double v = q->sigmav * c->vel_to_cgs;
+        v /= 1.0e5;
+        if (v < 1) v = 1;
+        double somer_v = c->c_in_cgs / v / 1.0e4;
+        return K * somer_v * q->dens * q->mass / (4.0 * ORC_PI * d * d);
+    }
+    case ORC_FLUX_ANN_SOMV2: {
+        /* flux_ann_somv2.cpp:21-41: (c / v)^2 / 1e8, v saturated at 5 km/s */
+        double v = q->sigmav * c->vel_to_cgs;
+        v /= 1.0e5;
+        if (v < 5) v = 5;
+        double somer_v = c->c_in_cgs / v;
+        somer_v *= (somer_v / 1.0e8);
+        return K * somer_v * q->dens * q->mass / (4.0 * ORC_PI * d * d);
+    }
+    case ORC_FLUX_DECAY:
+        /* flux_decay.cpp:10-16 */
+        return c->decay_factor * q->mass / (4.0 * ORC_PI * d * d);
+    case ORC_FLUX_ANN_ADIABATIC: {
+        /* flux_ann_adiabatic.cpp:11-30: profile factor of the distance to
+         * the halo centre in kpc/h */
+        double dx = q->posx - c->cpos[0];
+        double dy = q->posy - c->cpos[1];
+        double dz = q->posz - c->cpos[2];
+        double r = sqrt(dx * dx + dy * dy + dz * dz) * 40000.0;
+        double factor = (14.6534 * pow(1 + 0.0355872 * r, 1.76)) /
+                        (pow(1 + 0.10888 * pow(r, 0.76), 3.34211) * pow(r, 0.13));
+        double f = K * q->dens * q->mass / (4.0 * ORC_PI * d * d);
+        f *= (factor * factor);
+        return f;
+    }
+    default:
+        return orc_getflux(K, q, d);
+    }
 }
 
 void orc_calc_angles(double x, double y, double z, double d, const double rot[9],
@@ -580,7 +636,7 @@ static int64_t one_particle(const orc_particle *q, const orc_config *c, double t
     double d = sqrt(((double)q->posx - ox) * ((double)q->posx - ox) +
                     ((double)q->posy - oy) * ((double)q->posy - oy) +
                     ((double)q->posz - oz) * ((double)q->posz - oz));
-    double f = orc_getflux(c->flux_factor, q, d);
+    double f = orc_getflux_cfg(c, q, d);
     double costheta, phi;
     orc_calc_angles((double)q->posx - ox, (double)q->posy - oy, (double)q->posz - oz, d,
                     c->rotmatrix, &costheta, &phi);

@@ -73,12 +73,28 @@ typedef struct {
 } orc_params;
 
 /* Per-run constants the generator needs (subset of orc_params). */
+/* Flux plug-ins (the src/flux/ files, one linked per executable in the
+ * reference, src/Makefile:130-160). */
+enum {
+    ORC_FLUX_ANN_NOSOM = 0, /* flux_ann_nosom.cpp */
+    ORC_FLUX_ANN_PWAVE = 1, /* flux_ann_pwave.cpp */
+    ORC_FLUX_ANN_SOMV = 2,  /* flux_ann_somv.cpp */
+    ORC_FLUX_ANN_SOMV2 = 3, /* flux_ann_somv2.cpp */
+    ORC_FLUX_DECAY = 4,     /* flux_decay.cpp */
+    ORC_FLUX_ANN_ADIABATIC = 5 /* flux_ann_adiabatic.cpp */
+};
+
 typedef struct {
     double opos[3];
     double rotmatrix[9];
     double flux_factor;
     double mass_cut;     /* hires_particle_mass(float) * 1.1 (genmap.cpp:52-58,82) */
     int nside;
+    int flux_kind;       /* ORC_FLUX_* (default: nosom) */
+    double decay_factor; /* flux_decay.cpp:11-14 unit factor */
+    double vel_to_cgs;   /* codeunits.velocity_to_cgs */
+    double c_in_cgs;     /* natconst.c_in_cgs */
+    double cpos[3];      /* params.cpos (flux_ann_adiabatic.cpp:16-18) */
 } orc_config;
 
 typedef struct {
@@ -114,6 +130,8 @@ void orc_config_from_params(const orc_params *p, orc_config *c);
 
 /* ---- angles / flux ---- */
 double orc_getflux(double flux_factor, const orc_particle *q, double d);
+/* getflux of the configured plug-in (c->flux_kind). */
+double orc_getflux_cfg(const orc_config *c, const orc_particle *q, double d);
 void orc_calc_angles(double x, double y, double z, double d, const double rot[9],
                      double *costheta, double *phi);
 

@@ -47,9 +47,15 @@ class OrcParams(C.Structure):
     ]
 
 
+# flux plug-ins (the reference's src/flux/ files), OrcConfig.flux_kind
+FLUX_KINDS = ("ann_nosom", "ann_pwave", "ann_somv", "ann_somv2", "decay", "ann_adiabatic")
+
+
 class OrcConfig(C.Structure):
     _fields_ = [("opos", C.c_double * 3), ("rotmatrix", C.c_double * 9),
-                ("flux_factor", C.c_double), ("mass_cut", C.c_double), ("nside", C.c_int)]
+                ("flux_factor", C.c_double), ("mass_cut", C.c_double), ("nside", C.c_int),
+                ("flux_kind", C.c_int), ("decay_factor", C.c_double), ("vel_to_cgs", C.c_double),
+                ("c_in_cgs", C.c_double), ("cpos", C.c_double * 3)]
 
 
 class OrcStats(C.Structure):
@@ -96,6 +102,8 @@ def lib():
         L.orc_config_from_params.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcConfig)]
         L.orc_getflux.argtypes = [C.c_double, C.c_void_p, C.c_double]
         L.orc_getflux.restype = C.c_double
+        L.orc_getflux_cfg.argtypes = [C.POINTER(OrcConfig), C.c_void_p, C.c_double]
+        L.orc_getflux_cfg.restype = C.c_double
         L.orc_calc_angles.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_read_native.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.POINTER(C.c_void_p)]
@@ -127,6 +135,23 @@ def ref():
         R.ref_sizeof_particle.restype = C.c_int
         _ref = R
     return _ref
+
+
+_ref_flux = {}
+
+
+def ref_flux(kind: str):
+    """ref_flux(ini, rec12, d) of the reference's flux plug-in `kind`
+    (oracle/_ref/libref_flux_<kind>.so), or None when it was not built."""
+    if kind not in _ref_flux:
+        path = os.path.join(os.path.dirname(REF_PATH), f"libref_flux_{kind}.so")
+        R = None
+        if os.path.exists(path):
+            R = C.CDLL(path)
+            R.ref_flux.argtypes = [C.c_char_p, np.ctypeslib.ndpointer(np.float32, flags="C"), C.c_double]
+            R.ref_flux.restype = C.c_double
+        _ref_flux[kind] = R
+    return _ref_flux[kind]
 
 
 # ---------------------------------------------------------------- wrappers
@@ -200,6 +225,12 @@ def ang2pix(nside: int, theta: float, phi: float) -> int:
 def getflux(flux_factor: float, rec: np.ndarray, d: float) -> float:
     rec = np.ascontiguousarray(rec, dtype=PARTICLE_DTYPE)
     return float(lib().orc_getflux(flux_factor, rec.ctypes.data, d))
+
+
+def getflux_cfg(cfg: OrcConfig, rec: np.ndarray, d: float) -> float:
+    """getflux of the plug-in cfg.flux_kind (index into FLUX_KINDS)."""
+    rec = np.ascontiguousarray(rec, dtype=PARTICLE_DTYPE)
+    return float(lib().orc_getflux_cfg(C.byref(cfg), rec.ctypes.data, d))
 
 
 def calc_angles(x, y, z, d, rotmatrix):

@@ -8,6 +8,9 @@ Fixtures:
   params_kat.json   Parameters outputs (rotation matrix, observer, unit
                     chain, masses, map) and getflux values, full precision,
                     for the ini files below.
+  flux_kat.json     getflux of every flux plug-in (oracle/_ref/libref_flux_<kind>.so,
+                    one per src/flux/flux_<kind>.cpp) on records spanning the
+                    velocity saturation points, for two ini files.
   reader_kat.npz    a small native particle file and mask, and the records
                     DataReader returns for several (test, mem_buffer) settings.
   VL2_info.ini, test_VL2_info.ini
@@ -47,6 +50,40 @@ def write_synth_inis():
               mem_buffer=4096)
 
 
+def write_flux_kat(rng):
+    kinds = oracle.FLUX_KINDS
+    out = {"kinds": list(kinds), "cases": {}}
+    for ini in ("VL2_info.ini", "synth_offaxis.ini"):
+        path = os.path.join(HERE, ini)
+        p = oracle.params_from_ini(path)
+        n = 24
+        recs = np.zeros(n, dtype=oracle.PARTICLE_DTYPE)
+        recs["mass"] = rng.uniform(1e-9, 5e3, n)
+        recs["dens"] = rng.uniform(1e-3, 1e6, n)
+        recs["hsmooth"] = rng.uniform(0.01, 10, n)
+        # velocity dispersions from 0.1 to 100 km/s (both saturation points)
+        recs["sigmav"] = np.geomspace(0.1, 100.0, n) * 1e5 / p.velocity_to_cgs
+        off = rng.normal(size=(n, 3)) * np.geomspace(1e-4, 2.0, n)[:, None]
+        recs["posx"] = p.cpos[0] + off[:, 0]
+        recs["posy"] = p.cpos[1] + off[:, 1]
+        recs["posz"] = p.cpos[2] + off[:, 2]
+        dists = rng.uniform(0.05, 300.0, n)
+        case = {"records": recs.view(np.float32).reshape(n, 12).tolist(),
+                "dists": [float(d).hex() for d in dists], "flux": {}}
+        for k in kinds:
+            R = oracle.ref_flux(k)
+            if R is None:
+                sys.exit(f"oracle/_ref/libref_flux_{k}.so is not built")
+            vals = []
+            for r, d in zip(recs, dists):
+                f32 = np.ascontiguousarray(np.array([r], dtype=oracle.PARTICLE_DTYPE)).view(np.float32).copy()
+                vals.append(float(R.ref_flux(path.encode(), f32, float(d))).hex())
+            case["flux"][k] = vals
+        out["cases"][ini] = case
+    with open(os.path.join(HERE, "flux_kat.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def main():
     oracle.build()
     R = oracle.ref()
@@ -77,6 +114,7 @@ def main():
                              "flux": [float(f).hex() for f in flux]}
     with open(os.path.join(HERE, "params_kat.json"), "w") as fh:
         json.dump(out, fh, indent=1)
+    write_flux_kat(np.random.default_rng(777))
 
     # native reader KAT
     n = 1000

@@ -118,3 +118,24 @@ def test_host_footprint_edge_geometries(pw, orc, tmp_path):
             wn = w[o].astype(np.float64)
             wn /= wn.sum()
             assert np.max(np.abs(wn - ow) / ow) <= 5e-6
+
+
+def test_product_flux_plugins_match_oracle(pw, orc, tmp_path):
+    """The product's pre-pass flux (host build of the kernels' code) for every
+    plug-in equals the oracle's, which the reference's own plug-ins pin."""
+    ini = pw.write_ini(str(tmp_path / "o.ini"), nside=128, centerpos=(0.5, 0.25, -0.75))
+    p, _, _ = pw.params_from_ini(ini, "fp64")
+    cfg = orc.config_from_params(orc.params_from_ini(ini))
+    assert p.decay_factor == cfg.decay_factor and p.vel_to_cgs == cfg.vel_to_cgs
+    assert p.c_in_cgs == cfg.c_in_cgs and list(p.cpos) == list(cfg.cpos)
+    recs = pw.synth.to_records(pw.synth.make("nfw_1e5", n=300)).copy()
+    rng = np.random.default_rng(5)
+    recs["sigmav"] = np.geomspace(0.05, 300.0, len(recs)) * 1e5 / cfg.vel_to_cgs * rng.uniform(0.9, 1.1, len(recs))
+    for k, kind in enumerate(pw.capi.FLUX_KINDS):
+        pk = p.with_flux(kind)
+        cfg.flux_kind = k
+        for q in recs:
+            kind_p, _, _, fl = pw.capi.host_footprint(pk, q)
+            _, _, ofl = orc.particle_footprint(q, cfg)
+            if kind_p:
+                assert fl == ofl, kind

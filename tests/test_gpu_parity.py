@@ -159,3 +159,30 @@ def test_grouping_wide_windows(pw, orc, gpu, precision):
     print(rep)
     tol, ttol = TOL[precision]
     assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "compensated"])
+def test_flux_plugins(pw, orc, gpu, precision):
+    """Maps of every flux plug-in (the reference's src/flux/flux_<kind>.cpp)
+    against the oracle with the same plug-in, nside 64."""
+    import os
+    import tempfile
+    recs = pw.synth.to_records(pw.synth.make("nfw_1e5", n=20000)).copy()
+    with tempfile.TemporaryDirectory() as d:
+        ini = pw.write_ini(os.path.join(d, "o.ini"), nside=64, centerpos=(0.5, 0.25, -0.75))
+        cfg = orc.config_from_params(orc.params_from_ini(ini))
+        p, _, _ = pw.params_from_ini(ini, precision)
+    recs["sigmav"] = np.geomspace(0.05, 300.0, len(recs)) * 1e5 / cfg.vel_to_cgs
+    tol, ttol = TOL[precision]
+    for k, kind in enumerate(pw.capi.FLUX_KINDS):
+        print(f"case flux plug-in {kind} precision={precision}", flush=True)
+        cfg.flux_kind = k
+        ref, ost = orc.genmap(recs, cfg, nthreads=8)
+        with pw.Projector(p.with_flux(kind), len(recs)) as pr:
+            pr.render(recs)
+            m = pr.get_map()
+            st = pr.stats()
+        _check_counts(st, ost)
+        rep = rel_err_report(m, ref)
+        print(kind, rep)
+        assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol, kind
