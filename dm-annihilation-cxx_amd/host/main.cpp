@@ -36,11 +36,10 @@ int main(int argc, const char **argv)
     if (verbose) std::cout << "INFO FILE: " << info_file << std::endl;
     Parameters par(info_file);
     if (verbose) par.printParameters();
-    if (!par.isNative) {
-        fprintf(stderr, "Only native particle files are supported (datafile = ...).\n");
-        return 1;
-    }
-    DataReader *reader = new DataReader(par.nativeDatafile, par.isMask, par.maskFileName);
+    // src/main.cpp:41-53: native file, or tipsy + smooth side files
+    DataReader *reader = par.isNative ? new DataReader(par.nativeDatafile, par.isMask, par.maskFileName)
+                                      : new DataReader(par.partFile, par.densFile, par.hsmFile, par.dspFile,
+                                                       par.isMask, par.maskFileName);
     reader->setTest(par.testNum);
     reader->setBuf(par.memParts);
     reader->open();

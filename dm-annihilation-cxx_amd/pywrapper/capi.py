@@ -68,7 +68,7 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device",
            "dmann_batch_synchronize", "dmann_batch_npix", "dmann_batch_map_device", "dmann_batch_get_map",
            "dmann_batch_get_stats", "dmann_batch_graph_replays", "dmann_batch_destroy",
-           "dmann_batch_last_error")
+           "dmann_batch_last_error", "dmann_read_particles")
 
 _lib = None
 
@@ -103,6 +103,8 @@ def load_library():
     L.dmann_npix.restype = i64
     L.dmann_destroy.argtypes = [vp]
     L.dmann_destroy.restype = None
+    L.dmann_read_particles.argtypes = [C.c_char_p, vp, i64]
+    L.dmann_read_particles.restype = i64
     L.dmann_write_healpix_fits.argtypes = [C.c_char_p, vp, i64, C.c_int]
     L.dmann_read_healpix_fits.argtypes = [C.c_char_p, vp, i64, C.POINTER(C.c_int), C.POINTER(i64)]
     L.dmann_host_footprint.argtypes = [C.POINTER(DmannParams), vp, vp, vp, i64, C.POINTER(C.c_double),
@@ -313,6 +315,20 @@ class Projector:
             self.close()
         except Exception:
             pass
+
+
+def read_particles(ini: str) -> np.ndarray:
+    """The particles an ini file names (native or tipsy + smooth files, with
+    its test truncation and mask), as DMParticle records."""
+    L = load_library()
+    n = int(L.dmann_read_particles(ini.encode(), None, 0))
+    if n < 0:
+        raise DmannError(-n, f"cannot read the particles of {ini}")
+    out = np.zeros(n, dtype=PARTICLE_DTYPE)
+    m = int(L.dmann_read_particles(ini.encode(), C.c_void_p(out.ctypes.data), n))
+    if m != n:
+        raise DmannError(-m if m < 0 else 4, f"particle files of {ini} changed while read")
+    return out
 
 
 def host_footprint(params: DmannParams, rec):

@@ -238,6 +238,16 @@ int dmann_write_healpix_fits(const char *path, const double *map, int64_t npix, 
 /* Reads such a file; with map == NULL only *nside / *npix are filled. */
 int dmann_read_healpix_fits(const char *path, double *map, int64_t cap, int *nside, int64_t *npix);
 
+/* Reads the particles the ini file names -- the native file (`datafile`) or
+ * a tipsy file with its smooth side files (`partfile`, `densfile`,
+ * `hsmfile`, `vdspfile`) -- with the ini's `test` truncation and mask, in
+ * chunks of `mem_buffer`, exactly as the map generator reads them
+ * (src/datareader.cpp:23-298; the reference's pywrapper, src/pywrapper/,
+ * exposed this reader).  Copies up to cap records into out (may be NULL)
+ * and returns the number of particles, or -DMANN_ERR_ARG when a file is
+ * missing or malformed, -DMANN_ERR_DOMAIN when it ends early. */
+int64_t dmann_read_particles(const char *ini, dmann_particle *out, int64_t cap);
+
 /* Host-side footprint of one particle, computed with the same code the
  * kernels run (no GPU involved; medium-class particles with the grouped
  * kernel's ring terms and weight series, the others with the sky-covering

@@ -132,6 +132,9 @@ def ref():
         R.ref_read_native.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_char_p,
                                       np.ctypeslib.ndpointer(np.float32, flags="C"), C.c_longlong]
         R.ref_read_native.restype = C.c_longlong
+        R.ref_read_tipsy.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_char_p,
+                                     np.ctypeslib.ndpointer(np.float32, flags="C"), C.c_longlong]
+        R.ref_read_tipsy.restype = C.c_longlong
         R.ref_sizeof_particle.restype = C.c_int
         _ref = R
     return _ref

@@ -72,6 +72,29 @@ double ref_getflux(const char *ini, const float *rec12, double d)
     return getflux(&par, p, d);
 }
 
+// Reads a tipsy file + smooth side files (density, hsmooth, sigma-v) through
+// DataReader's tipsy constructor and chunked buffer, applying `test` and the
+// optional mask, into out (cap records of 12 floats).  Returns the count.
+long long ref_read_tipsy(const char *part, const char *dens, const char *hsm, const char *sig, int test,
+                         int membuf, const char *mask, float *out, long long cap)
+{
+    const bool use_mask = mask && *mask;
+    DataReader r(std::string(part), std::string(dens), std::string(hsm), std::string(sig), use_mask,
+                 std::string(use_mask ? mask : ""));
+    r.setTest(test);
+    r.setBuf(membuf);
+    r.open();
+    long long n = 0;
+    while (r.hasNext()) {
+        const int m = r.getMemparts();
+        for (int i = 0; i < m; i++, n++)
+            if (n < cap) std::memcpy(out + 12 * n, r.getBuf() + i, sizeof(DMParticle));
+        r.loadBuffer();
+    }
+    r.close();
+    return n;
+}
+
 // Reads a native particle file through DataReader's chunked buffer
 // (membuf particles per chunk), applying `test` and the optional mask,
 // into out (cap records of 12 floats).  Returns the number of records.

@@ -11,6 +11,10 @@ Fixtures:
   flux_kat.json     getflux of every flux plug-in (oracle/_ref/libref_flux_<kind>.so,
                     one per src/flux/flux_<kind>.cpp) on records spanning the
                     velocity saturation points, for two ini files.
+  tipsy_kat.npz     the bytes of a small tipsy XDR file (gas, dark, star
+                    records), its smooth side files and a mask, and the
+                    records DataReader's tipsy path returns for several
+                    (test, mem_buffer, mask) settings.
   reader_kat.npz    a small native particle file and mask, and the records
                     DataReader returns for several (test, mem_buffer) settings.
   VL2_info.ini, test_VL2_info.ini
@@ -84,6 +88,49 @@ def write_flux_kat(rng):
         json.dump(out, fh, indent=1)
 
 
+def tipsy_files(rng, nsph=7, ndark=1234, nstar=5):
+    """Bytes of a tipsy XDR file (big-endian: header, gas, dark, star
+    records) and of its three smooth side files (int32 nbodies + one float
+    per body, host byte order)."""
+    import struct
+    nbodies = nsph + ndark + nstar
+    head = struct.pack(">diiiiii", 0.375, nbodies, 3, nsph, ndark, nstar, 0)
+    gas = rng.standard_normal((nsph, 12)).astype(">f4")
+    dark = (rng.standard_normal((ndark, 9)) * np.arange(1, 10)).astype(">f4")
+    dark[:, 0] = np.abs(dark[:, 0]) + 1.0
+    star = rng.standard_normal((nstar, 11)).astype(">f4")
+    part = head + gas.tobytes() + dark.tobytes() + star.tobytes()
+    side = []
+    for scale in (1e3, 0.1, 30.0):
+        vals = (np.abs(rng.standard_normal(nbodies)) * scale).astype("<f4")
+        side.append(np.array([nbodies], dtype="<i4").tobytes() + vals.tobytes())
+    mask = rng.random(ndark) > 0.25
+    mbytes = np.array([ndark], dtype="<i4").tobytes() + mask.astype(np.uint8).tobytes()
+    return part, side, mbytes
+
+
+def write_tipsy_kat(R, rng):
+    part, side, mbytes = tipsy_files(rng)
+    names = ["_t.tipsy", "_t.den32", "_t.hsm32", "_t.sigma32", "_t.mask"]
+    paths = [os.path.join(HERE, nm) for nm in names]
+    for pth, b in zip(paths, [part] + side + [mbytes]):
+        with open(pth, "wb") as fh:
+            fh.write(b)
+    out = {"part": np.frombuffer(part, np.uint8), "dens": np.frombuffer(side[0], np.uint8),
+           "hsm": np.frombuffer(side[1], np.uint8), "sigma": np.frombuffer(side[2], np.uint8),
+           "mask": np.frombuffer(mbytes, np.uint8)}
+    for name, test, membuf, use_mask in (("all_chunk100", -1, 100, False), ("test500_chunk64", 500, 64, False),
+                                         ("mask_chunk333", -1, 333, True), ("mask_test900_chunk2000", 900, 2000, True)):
+        buf = np.zeros(2000 * 12, dtype=np.float32)
+        got = R.ref_read_tipsy(paths[0].encode(), paths[1].encode(), paths[2].encode(), paths[3].encode(), test,
+                               membuf, (paths[4] if use_mask else "").encode(), buf, 2000)
+        out[f"{name}__out"] = buf[:got * 12].reshape(got, 12).copy()
+        out[f"{name}__cfg"] = np.array([test, membuf, int(use_mask)])
+    for pth in paths:
+        os.unlink(pth)
+    np.savez_compressed(os.path.join(HERE, "tipsy_kat.npz"), **out)
+
+
 def main():
     oracle.build()
     R = oracle.ref()
@@ -115,6 +162,7 @@ def main():
     with open(os.path.join(HERE, "params_kat.json"), "w") as fh:
         json.dump(out, fh, indent=1)
     write_flux_kat(np.random.default_rng(777))
+    write_tipsy_kat(R, np.random.default_rng(4242))
 
     # native reader KAT
     n = 1000

@@ -1,8 +1,10 @@
 """Particle-file reader and FITS output.
 
-Reader: the oracle's native-format reader against records that the
-reference's own DataReader (src/datareader.cpp) returned for the same file
-(tests/golden/reader_kat.npz): chunked buffers, `test` truncation, mask.
+Reader: the oracle's native-format reader and the library's reader (native
+and tipsy + smooth side files) against records that the reference's own
+DataReader (src/datareader.cpp) returned for the same bytes
+(tests/golden/reader_kat.npz, tipsy_kat.npz): chunked buffers, `test`
+truncation, mask.
 FITS: the library's HEALPix FITS writer against the FITS standard (2880-byte
 blocks, header keywords, big-endian float64 column) and a round trip.
 """
@@ -79,3 +81,62 @@ def test_fits_layout_and_round_trip(pw, tmp_path):
     back, ns = pw.read_fits(path)
     assert ns == 2 and np.array_equal(back, m[:48])
     assert struct.unpack(">d", open(path, "rb").read()[5760:5768])[0] == m[0]
+
+
+@pytest.mark.parametrize("case", ["all_chunk97", "test333_chunk50", "mask_chunk128", "mask_test700_chunk1000"])
+def test_product_reader_native_matches_reference_datareader(pw, kat, tmp_path, case):
+    """The library's DataReader (C ABI dmann_read_particles) on the native
+    format against the records the reference's DataReader returned."""
+    test, membuf, use_mask = (int(x) for x in kat[f"{case}__cfg"])
+    f = str(tmp_path / "p.bin")
+    m = str(tmp_path / "m.bin")
+    _write_native(f, kat["particles"], m, kat["mask"])
+    extra = {"datafile": f, "test": test}
+    if use_mask:
+        extra["mask"] = m
+    ini = pw.write_ini(str(tmp_path / "r.ini"), nside=16, mem_buffer=membuf, extra=extra)
+    got = pw.read_particles(ini)
+    assert np.array_equal(got.view(np.float32).reshape(-1, 12), kat[f"{case}__out"])
+
+
+@pytest.fixture(scope="module")
+def tipsy_kat():
+    return np.load(os.path.join(GOLD, "tipsy_kat.npz"))
+
+
+@pytest.mark.parametrize("case", ["all_chunk100", "test500_chunk64", "mask_chunk333", "mask_test900_chunk2000"])
+def test_product_reader_tipsy_matches_reference_datareader(pw, tipsy_kat, tmp_path, case):
+    """The tipsy path (XDR particle file with gas and star records around the
+    dark ones, smooth side files, mask) against the reference's DataReader
+    on the same bytes."""
+    test, membuf, use_mask = (int(x) for x in tipsy_kat[f"{case}__cfg"])
+    paths = {}
+    for key in ("part", "dens", "hsm", "sigma", "mask"):
+        paths[key] = str(tmp_path / f"t.{key}")
+        tipsy_kat[key].tofile(paths[key])
+    extra = {"partfile": paths["part"], "densfile": paths["dens"], "hsmfile": paths["hsm"],
+             "vdspfile": paths["sigma"], "test": test}
+    if use_mask:
+        extra["mask"] = paths["mask"]
+    ini = pw.write_ini(str(tmp_path / "r.ini"), nside=16, mem_buffer=membuf, extra=extra)
+    got = pw.read_particles(ini)
+    assert np.array_equal(got.view(np.float32).reshape(-1, 12), tipsy_kat[f"{case}__out"])
+
+
+def test_product_reader_errors(pw, tipsy_kat, tmp_path):
+    paths = {}
+    for key in ("part", "dens", "hsm", "sigma"):
+        paths[key] = str(tmp_path / f"t.{key}")
+        tipsy_kat[key].tofile(paths[key])
+    # a side file with the wrong body count
+    bad = tipsy_kat["dens"].copy()
+    bad[:4] = np.frombuffer(np.array([7], dtype="<i4").tobytes(), np.uint8)
+    bad.tofile(paths["dens"])
+    extra = {"partfile": paths["part"], "densfile": paths["dens"], "hsmfile": paths["hsm"],
+             "vdspfile": paths["sigma"]}
+    ini = pw.write_ini(str(tmp_path / "r.ini"), nside=16, extra=extra)
+    with pytest.raises(pw.DmannError):
+        pw.read_particles(ini)
+    ini = pw.write_ini(str(tmp_path / "n.ini"), nside=16, extra={"datafile": str(tmp_path / "missing.bin")})
+    with pytest.raises(pw.DmannError):
+        pw.read_particles(ini)
