@@ -387,19 +387,32 @@ DMANN_HD float weight32(float x, float A, float B, float Ac, float D, float K4)
     }
 }
 
-// The fast form with the series lengths of fast_variant: PV 2 is
-// weight32<0>; PV 0 and 1 drop the terms below 2e-8 relative.
+// The fast form with the series lengths of fast_variant (PV 2 is
+// weight32<0>; PV 0 and 1 drop the terms below 2e-8 relative), with the
+// sin^2 coefficients folded into the ring's B once per ring segment
+// (ring_coef), so a pixel costs 6 VALU operations + exp2 in the 2+2 form:
+//   h = A + u (B/4 - u B/48 [+ u^2 B/1440]),  u = x^2.
+struct RingCoef {
+    float A, c1, c2, c3;
+};
+DMANN_HD RingCoef ring_coef(float A, float B)
+{
+    RingCoef c;
+    c.A = A;
+    c.c1 = B * 0.25f;
+    c.c2 = B * (-1.0f / 48.0f);
+    c.c3 = B * (1.0f / 1440.0f);
+    return c;
+}
 template <int PV>
-DMANN_HD float weight_fast(float x, float A, float B, float K4)
+DMANN_HD float weight_coef(float x, const RingCoef &c, float K4)
 {
     if constexpr (PV == 2) {
-        return weight32<0>(x, A, B, 0.0f, 0.0f, K4);
+        return weight32<0>(x, c.A, c.c1 * 4.0f, 0.0f, 0.0f, K4);
     } else {
         const float u = x * x;
-        float p = PV == 0 ? -1.0f / 48.0f : fmaf(u, 1.0f / 1440.0f, -1.0f / 48.0f);
-        p = fmaf(u, p, 0.25f);
-        const float h = fmaf(B, u * p, A);
-        // K4 (psi/2)^2 = h (K4 + K4 h/3 [+ K4 8h^2/45])
+        const float p = PV == 0 ? fmaf(u, c.c2, c.c1) : fmaf(u, fmaf(u, c.c3, c.c2), c.c1);
+        const float h = fmaf(u, p, c.A);
         float gq = PV == 0 ? K4 * (1.0f / 3.0f) : fmaf(h, K4 * (8.0f / 45.0f), K4 * (1.0f / 3.0f));
         gq = fmaf(h, gq, K4);
         return fast_exp2(h * gq);
@@ -424,10 +437,11 @@ DMANN_HD float seg_x(const RingLite &rl, int t)
 DMANN_HD float lite_weight32(const PGeo &g, const RingLite &rl, int t)
 {
     const float x = seg_x(rl, t);
+    const RingCoef c = ring_coef(rl.A, rl.B);
     switch (g.pv) {
-    case 0: return weight_fast<0>(x, rl.A, rl.B, g.K4);
-    case 1: return weight_fast<1>(x, rl.A, rl.B, g.K4);
-    default: return weight_fast<2>(x, rl.A, rl.B, g.K4);
+    case 0: return weight_coef<0>(x, c, g.K4);
+    case 1: return weight_coef<1>(x, c, g.K4);
+    default: return weight_coef<2>(x, c, g.K4);
     }
 }
 
@@ -435,12 +449,9 @@ DMANN_HD float ring_weight32(const PGeo &g, const RingRec &rr, int t)
 {
     const float kf = (float)(t - rr.kc_rel);
     const float x = fmaf(kf, rr.step, rr.offm);
+    // the sky-covering kernel's forms: full series, or the general form
     if (g.mode != 0) return weight32<1>(x, rr.A, rr.E, rr.Ac, rr.D, g.K4);
-    switch (g.pv) {
-    case 0: return weight_fast<0>(x, rr.A, rr.B, g.K4);
-    case 1: return weight_fast<1>(x, rr.A, rr.B, g.K4);
-    default: return weight_fast<2>(x, rr.A, rr.B, g.K4);
-    }
+    return weight32<0>(x, rr.A, rr.B, 0.0f, 0.0f, g.K4);
 }
 
 } // namespace dmann

@@ -169,7 +169,7 @@ __device__ __forceinline__ void deposit_direct(const GroupLDS &S, int q, int t0,
 #pragma unroll
     for (int t = 0; t < kSeg; ++t) {
         if (t < len) {
-            const float v = weight_fast<PV>(fmaf((float)t, step, x0), A, B, K4) * sc;
+            const float v = weight_coef<PV>(fmaf((float)t, step, x0), ring_coef(A, B), K4) * sc;
             int col = d.x + t0 + t;
             if (col < 0) col += nr;
             else if (col >= nr) col -= nr;
@@ -205,10 +205,11 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
             const int t0 = (s - r.z) * kSeg;
             const int len = r.y - t0 < kSeg ? r.y - t0 : kSeg;
             const float x0 = fmaf((float)(t0 - r.x), f.w, f.z);
+            const RingCoef rc = ring_coef(f.x, f.y);
             if constexpr (MODE == 0) {
 #pragma unroll
                 for (int t = 0; t < kSeg; ++t) {
-                    const float w = weight_fast<PV>(fmaf((float)t, f.w, x0), f.x, f.y, K4);
+                    const float w = weight_coef<PV>(fmaf((float)t, f.w, x0), rc, K4);
                     part += t < len ? w : 0.0f;
                 }
             } else if constexpr (WINDOW) {
@@ -219,7 +220,7 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
                     float *a = &S.acc[w.x + off];
 #pragma unroll
                     for (int t = 0; t < kSeg; ++t) {
-                        const float v = weight_fast<PV>(fmaf((float)t, f.w, x0), f.x, f.y, K4) * sc;
+                        const float v = weight_coef<PV>(fmaf((float)t, f.w, x0), rc, K4) * sc;
                         a[t] += t < len ? v : 0.0f;
                     }
                 } else {
