@@ -120,6 +120,8 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
             return bfail(nullptr, DMANN_ERR_ARG, "map %d: all maps of a batch share one precision", m);
         if (params[m].flux_kind < DMANN_FLUX_ANN_NOSOM || params[m].flux_kind > DMANN_FLUX_ANN_ADIABATIC)
             return bfail(nullptr, DMANN_ERR_ARG, "map %d: unknown flux plug-in %d", m, params[m].flux_kind);
+        if (!(params[m].footprint_pad >= 0.0 && params[m].footprint_pad < kPi))
+            return bfail(nullptr, DMANN_ERR_ARG, "map %d: footprint_pad outside [0, pi)", m);
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(nullptr, DMANN_ERR_HIP, "no HIP device");

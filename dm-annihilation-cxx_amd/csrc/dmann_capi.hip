@@ -145,6 +145,7 @@ void fill_devparams(const dmann_params *p, DevParams &P, int large_pix)
     P.vel_to_cgs = p->vel_to_cgs;
     P.c_in_cgs = p->c_in_cgs;
     for (int i = 0; i < 3; i++) P.cpos[i] = p->cpos[i];
+    P.footprint_pad = p->footprint_pad;
 }
 } // namespace dmann
 
@@ -158,6 +159,8 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
         return fail(nullptr, DMANN_ERR_ARG, "nside %d outside 1..%d", p->nside, kMaxNside);
     if (p->flux_kind < DMANN_FLUX_ANN_NOSOM || p->flux_kind > DMANN_FLUX_ANN_ADIABATIC)
         return fail(nullptr, DMANN_ERR_ARG, "unknown flux plug-in %d", p->flux_kind);
+    if (!(p->footprint_pad >= 0.0 && p->footprint_pad < kPi))
+        return fail(nullptr, DMANN_ERR_ARG, "footprint_pad %g outside [0, pi)", p->footprint_pad);
     if (max_parts < 0 || max_parts > (int64_t)INT32_MAX * 2)
         return fail(nullptr, DMANN_ERR_ARG, "max_parts %lld outside 0..%lld", (long long)max_parts,
                     (long long)INT32_MAX * 2);
@@ -412,7 +415,7 @@ int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)
     bool bad = false;
     const int cls = prepass_one(P, *reinterpret_cast<const Particle *>(q), rec, bad);
     if (bad || cls != kClsMedium) return -2;
-    const PGeo g = particle_geo(rec, P.nside);
+    const PGeo g = particle_geo(rec, P.nside, P.footprint_pad);
     const RingGeo G = ring_geo(P.nside);
     int64_t mism = 0;
     for (int gi = 0; gi < g.npart; ++gi) {
@@ -451,7 +454,7 @@ int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int
         if (cap > 0 && w) w[0] = 1.0f;
         return 1;
     }
-    const PGeo g = particle_geo(rec, P.nside);
+    const PGeo g = particle_geo(rec, P.nside, P.footprint_pad);
     int64_t k = 0;
     if (cls == kClsMedium) {
         // the grouped kernel's per-ring terms and weights

@@ -107,7 +107,7 @@ DMANN_HD int prepass_one(const DevParams &P, const Particle &q, DiscRec &rec, bo
         rec.cls = kClsPoint;
         return kClsPoint;
     }
-    const double rad = 2.0 * r;
+    const double rad = 2.0 * r + P.footprint_pad;
     const double est = (rad >= kPi) ? (double)P.npix : kTwoPi * (1.0 - cos(rad)) / P.dOmega;
     rec.est = est > 2.0e9 ? 2000000000 : (int)est;
     const double z0 = cos(theta);
@@ -142,13 +142,14 @@ DMANN_HD int fast_variant(double rad, double xmax)
     return 2;
 }
 
-DMANN_HD PGeo particle_geo(const DiscRec &rec, int nside)
+// pad: radians added to the footprint radius 2r (DevParams::footprint_pad).
+DMANN_HD PGeo particle_geo(const DiscRec &rec, int nside, double pad)
 {
     PGeo g;
     g.theta = rec.theta;
     g.phi = fmodulo(rec.phi, kTwoPi); // pointing::normalize
     g.r = rec.r;
-    g.rad = 2.0 * rec.r;
+    g.rad = 2.0 * rec.r + pad;
     g.flux = rec.flux;
     g.p0 = rec.p0;
     g.all = g.rad >= kPi;

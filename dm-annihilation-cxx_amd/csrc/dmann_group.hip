@@ -311,7 +311,7 @@ __device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const 
 
 template <int ACC>
 __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRec *__restrict__ recs, const unsigned *__restrict__ idx,
-                                                   const unsigned *__restrict__ counts, int nside, int K, MapRef map,
+                                                   const unsigned *__restrict__ counts, int nside, double pad, int K, MapRef map,
                                                    unsigned long long *__restrict__ stats, unsigned *__restrict__ work,
                                                    float4 *__restrict__ tab_f, int4 *__restrict__ tab_i)
 {
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
         double my_phi = 0.0, my_dphi = 0.0, my_z0 = 0.0, my_s0 = 0.0, my_xa = 0.0, my_cosr = 0.0, my_flux = 0.0;
         bool my_on = false;
         if (lane < nm) {
-            const PGeo mg = particle_geo(recs[idx[m0 + lane]], nside);
+            const PGeo mg = particle_geo(recs[idx[m0 + lane]], nside, pad);
             my_lo = mg.irmin;
             my_hi = mg.irmax;
             my_phi = mg.phi;
@@ -611,7 +611,8 @@ __global__ void k_zero_u32(unsigned *p, int n)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, const GroupCfg &gc, MapRef map,
+hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, double pad, const GroupCfg &gc,
+                        MapRef map,
                         unsigned long long *stats, hipStream_t st, hipEvent_t sorted)
 {
     k_zero_u32<<<1, 64, 0, st>>>(gc.work, 1);
@@ -625,10 +626,10 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
     k_sort_scatter<<<gc.sort_blocks, 256, 0, st>>>(gc.keys, counts, gc.hist, gc.idx);
     if (sorted) (void)hipEventRecord(sorted, st);
     if (acc == kAccComp)
-        k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work,
+        k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, pad, gc.k, map, stats, gc.work,
                                                                gc.tab_f, gc.tab_i);
     else
-        k_disc_group<kAccF32><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, gc.k, map, stats, gc.work,
+        k_disc_group<kAccF32><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, pad, gc.k, map, stats, gc.work,
                                                               gc.tab_f, gc.tab_i);
     return hipGetLastError();
 }

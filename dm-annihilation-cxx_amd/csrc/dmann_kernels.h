@@ -55,6 +55,7 @@ struct DevParams {
     int flux_kind;  // dmann_flux
     double decay_factor, vel_to_cgs, c_in_cgs;
     double cpos[3];
+    double footprint_pad; // radians added to 2r for pixel membership
 };
 
 // Device constants of one map from the C-ABI parameters.
@@ -97,7 +98,8 @@ struct GroupCfg {
     int4 *tab_i;
 };
 
-hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, const GroupCfg &gc, MapRef map,
+hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, double pad, const GroupCfg &gc,
+                        MapRef map,
                         unsigned long long *stats, hipStream_t st,
                         hipEvent_t sorted = nullptr);
 

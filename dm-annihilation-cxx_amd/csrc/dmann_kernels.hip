@@ -248,7 +248,7 @@ __device__ int render_particle_wave(const PGeo &g, int nside, WinLDS &L, int lan
 // One wavefront per particle of the medium class (grid-stride over waves).
 template <int ACC>
 __global__ __launch_bounds__(256) void k_disc_medium(const DiscRec *__restrict__ recs,
-                                                     const unsigned *__restrict__ counts, int nside,
+                                                     const unsigned *__restrict__ counts, int nside, double pad,
                                                      MapRef map, unsigned long long *__restrict__ stats)
 {
     __shared__ WinLDS lds[4];
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void k_disc_medium(const DiscRec *__restrict__
     const unsigned nwaves = gridDim.x * 4;
     unsigned long long n_ext = 0, n_fb = 0, n_contrib = 0;
     for (unsigned it = blockIdx.x * 4 + wid; it < count; it += nwaves) {
-        const PGeo g = particle_geo(recs[it], nside);
+        const PGeo g = particle_geo(recs[it], nside, pad);
         // the pre-pass only puts fast-form particles in this class
         const int T = render_particle_wave<ACC, 0>(g, nside, L, lane, map);
         if (T < 2) n_fb++;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_disc_medium(const DiscRec *__restrict__
 // walks whole rings (ring-major) from 64-ring windows, FP64 normalisation.
 template <int ACC>
 __global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ recs, int64_t n,
-                                                    const unsigned *__restrict__ counts, int nside,
+                                                    const unsigned *__restrict__ counts, int nside, double pad,
                                                     MapRef map, unsigned long long *__restrict__ stats)
 {
     __shared__ WinLDS lds[4];
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ 
     WinLDS &L = lds[wid];
     const unsigned count = counts[1];
     for (unsigned it = blockIdx.x; it < count; it += gridDim.x) {
-        const PGeo g = particle_geo(recs[n - 1 - (int64_t)it], nside);
+        const PGeo g = particle_geo(recs[n - 1 - (int64_t)it], nside, pad);
         const int nwin = (g.nrings + 63) / 64;
         const double vx = sin(g.theta) * cos(g.phi), vy = sin(g.theta) * sin(g.phi);
         double scale64 = 0.0;
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const Particle *__restrict__ pa
             int T = 0;
             double r = rec.r;
             if (cls != kClsPoint) {
-                const PGeo g = particle_geo(rec, P.nside);
+                const PGeo g = particle_geo(rec, P.nside, P.footprint_pad);
                 const double st = sin(g.theta);
                 const double v[3] = {st * cos(g.phi), st * sin(g.phi), g.z0};
                 const int nwin = (g.nrings + 63) / 64;
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256) void k_batch(const Particle *__restrict__ part
                 c_sub++;
                 continue;
             }
-            const PGeo g = particle_geo(rec, P.nside);
+            const PGeo g = particle_geo(rec, P.nside, P.footprint_pad);
             const int T = g.mode == 0 ? render_particle_wave<ACC, 0>(g, P.nside, L, lane, map)
                                       : render_particle_wave<ACC, 1>(g, P.nside, L, lane, map);
             if (T < 2) c_fb++;
@@ -636,14 +636,15 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
     if (pre_blocks > 0)
         k_prepass<ACC><<<pre_blocks, 256, 0, st>>>(d_parts, n, P, recs, counts, map, stats);
     if (ev) (void)hipEventRecord(ev[1], st);
-    k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, map, stats);
+    k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, P.footprint_pad, map, stats);
     if (ev) (void)hipEventRecord(ev[2], st);
     if (ACC != kAccF64 && gc && gc->k > 0) {
-        const hipError_t e = launch_group(ACC, recs, counts, P.nside, *gc, map, stats, st, ev ? ev[3] : nullptr);
+        const hipError_t e =
+            launch_group(ACC, recs, counts, P.nside, P.footprint_pad, *gc, map, stats, st, ev ? ev[3] : nullptr);
         if (e != hipSuccess) return e;
     } else {
         if (ev) (void)hipEventRecord(ev[3], st);
-        k_disc_medium<ACC><<<cfg.medium_blocks, 256, 0, st>>>(recs, counts, P.nside, map, stats);
+        k_disc_medium<ACC><<<cfg.medium_blocks, 256, 0, st>>>(recs, counts, P.nside, P.footprint_pad, map, stats);
     }
     if (ev) (void)hipEventRecord(ev[4], st);
     return hipGetLastError();

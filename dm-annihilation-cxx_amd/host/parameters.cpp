@@ -283,6 +283,7 @@ void Parameters::toDmannParams(dmann_params *p, int precision) const
     p->vel_to_cgs = codeunits.velocity_to_cgs;
     p->c_in_cgs = natconst.c_in_cgs;
     for (int i = 0; i < 3; i++) p->cpos[i] = params.cpos[i];
+    p->footprint_pad = 0.0;
 }
 
 static std::string join(const double *d, int n)

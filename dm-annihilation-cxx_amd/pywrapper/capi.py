@@ -8,6 +8,7 @@ runs on torch's current stream.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import tempfile
 
@@ -37,7 +38,14 @@ class DmannParams(C.Structure):
                 ("nside", C.c_int32), ("precision", C.c_int32),
                 ("flux_kind", C.c_int32), ("reserved", C.c_int32),
                 ("decay_factor", C.c_double), ("vel_to_cgs", C.c_double), ("c_in_cgs", C.c_double),
-                ("cpos", C.c_double * 3)]
+                ("cpos", C.c_double * 3), ("footprint_pad", C.c_double)]
+
+    def with_cuda_cutoff(self) -> "DmannParams":
+        """A copy whose footprints take every pixel within 2r + theta_pix/2,
+        the reference CUDA render's cutoff (theta_pix = 2 pi / (4 nside))."""
+        p = self.copy()
+        p.footprint_pad = math.pi / (4 * self.nside)
+        return p
 
     def with_flux(self, kind: str) -> "DmannParams":
         """A copy that uses the flux plug-in `kind` (one of FLUX_KINDS)."""

@@ -76,7 +76,12 @@ enum dmann_status {
  *  decay_factor  unit factor of src/flux/flux_decay.cpp:11-14
  *  vel_to_cgs    codeunits.velocity_to_cgs (pwave, somv, somv2)
  *  c_in_cgs      natconst.c_in_cgs
- *  cpos          params.cpos, the halo centre (adiabatic)                    */
+ *  cpos          params.cpos, the halo centre (adiabatic)
+ *  footprint_pad  radians added to the footprint radius 2r for pixel
+ *             membership (weights unchanged): 0, the default, is the CPU
+ *             generator's query_disc(2r) (src/genmap.cpp:119); pi/(4 nside)
+ *             = theta_pix/2 reproduces the reference CUDA render's cutoff
+ *             2r + theta_pix/2 (src/chealpixrender.cu:188).                */
 typedef struct dmann_params {
     double opos[3];
     double rotmatrix[9];
@@ -90,6 +95,7 @@ typedef struct dmann_params {
     double vel_to_cgs;
     double c_in_cgs;
     double cpos[3];
+    double footprint_pad;
 } dmann_params;
 
 /* Flux plug-ins: K rho m / (4 pi d^2) times

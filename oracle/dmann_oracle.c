@@ -652,10 +652,10 @@ static int64_t one_particle(const orc_particle *q, const orc_config *c, double t
     *pix0 = pix;
     *flux = f;
     if (2.0 * r < theta0) { *kind = 1; return 1; }
-    int64_t cnt = orc_query_disc(c->nside, theta, phi, 2.0 * r, sc->pix, sc->cap);
+    int64_t cnt = orc_query_disc(c->nside, theta, phi, 2.0 * r + c->footprint_pad, sc->pix, sc->cap);
     if (cnt > sc->cap) {
         if (scratch_reserve(sc, cnt)) return -1;
-        cnt = orc_query_disc(c->nside, theta, phi, 2.0 * r, sc->pix, sc->cap);
+        cnt = orc_query_disc(c->nside, theta, phi, 2.0 * r + c->footprint_pad, sc->pix, sc->cap);
     }
     if (cnt < 2) { *kind = 2; return 1; }
     double norm = 0.0;

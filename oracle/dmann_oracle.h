@@ -95,6 +95,9 @@ typedef struct {
     double vel_to_cgs;   /* codeunits.velocity_to_cgs */
     double c_in_cgs;     /* natconst.c_in_cgs */
     double cpos[3];      /* params.cpos (flux_ann_adiabatic.cpp:16-18) */
+    double footprint_pad; /* added to the query_disc radius 2r (0 = genmap.cpp:119;
+                             theta_pix/2 = the reference CUDA cutoff,
+                             chealpixrender.cu:188) */
 } orc_config;
 
 typedef struct {

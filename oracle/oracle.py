@@ -55,7 +55,7 @@ class OrcConfig(C.Structure):
     _fields_ = [("opos", C.c_double * 3), ("rotmatrix", C.c_double * 9),
                 ("flux_factor", C.c_double), ("mass_cut", C.c_double), ("nside", C.c_int),
                 ("flux_kind", C.c_int), ("decay_factor", C.c_double), ("vel_to_cgs", C.c_double),
-                ("c_in_cgs", C.c_double), ("cpos", C.c_double * 3)]
+                ("c_in_cgs", C.c_double), ("cpos", C.c_double * 3), ("footprint_pad", C.c_double)]
 
 
 class OrcStats(C.Structure):

@@ -139,3 +139,34 @@ def test_product_flux_plugins_match_oracle(pw, orc, tmp_path):
             _, _, ofl = orc.particle_footprint(q, cfg)
             if kind_p:
                 assert fl == ofl, kind
+
+
+def test_host_footprint_cuda_cutoff(pw, orc, tmp_path):
+    """With footprint_pad = theta_pix/2 (the reference CUDA render's cutoff
+    2r + theta_pix/2, src/chealpixrender.cu:188) the footprint is the set of
+    pixel centres within that radius -- equal to the oracle's with the same
+    pad, and a superset of the CPU generator's."""
+    import math
+    nside = 128
+    ini = pw.write_ini(str(tmp_path / "o.ini"), nside=nside)
+    p, _, _ = pw.params_from_ini(ini, "fp32")
+    pc = p.with_cuda_cutoff()
+    assert pc.footprint_pad == math.pi / (4 * nside)
+    cfg = orc.config_from_params(orc.params_from_ini(ini))
+    cfg.footprint_pad = pc.footprint_pad
+    cfg0 = orc.config_from_params(orc.params_from_ini(ini))
+    recs = pw.synth.to_records(pw.synth.make("nfw_1e5", n=400))
+    grew = 0
+    for q in recs:
+        kind, pix, w, fl = pw.capi.host_footprint(pc, q)
+        opix, ow, ofl = orc.particle_footprint(q, cfg)
+        assert np.array_equal(np.sort(pix), opix)
+        if kind == 3:
+            o = np.argsort(pix)
+            wn = w[o].astype(np.float64)
+            wn /= wn.sum()
+            assert np.max(np.abs(wn - ow) / ow) <= 5e-6
+            opix0, _, _ = orc.particle_footprint(q, cfg0)
+            assert set(opix0.tolist()) <= set(opix.tolist())
+            grew += len(opix) > len(opix0)
+    assert grew > 0

@@ -186,3 +186,30 @@ def test_flux_plugins(pw, orc, gpu, precision):
         rep = rel_err_report(m, ref)
         print(kind, rep)
         assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol, kind
+
+
+@pytest.mark.parametrize("precision", ["fp64", "compensated"])
+def test_cuda_cutoff(pw, orc, gpu, precision):
+    """Footprints of radius 2r + theta_pix/2 (the reference CUDA render's
+    cutoff, footprint_pad = pi / (4 nside)) against the oracle with the same
+    pad, nside 128."""
+    import os
+    import tempfile
+    print(f"case cuda cutoff precision={precision}", flush=True)
+    recs = pw.synth.to_records(pw.synth.make("nfw_1e5", n=20000))
+    with tempfile.TemporaryDirectory() as d:
+        ini = pw.write_ini(os.path.join(d, "o.ini"), nside=128)
+        cfg = orc.config_from_params(orc.params_from_ini(ini))
+        p, _, _ = pw.params_from_ini(ini, precision)
+    p = p.with_cuda_cutoff()
+    cfg.footprint_pad = p.footprint_pad
+    ref, ost = orc.genmap(recs, cfg, nthreads=8)
+    with pw.Projector(p, len(recs)) as pr:
+        pr.render(recs)
+        m = pr.get_map()
+        st = pr.stats()
+    _check_counts(st, ost)
+    rep = rel_err_report(m, ref)
+    print(rep)
+    tol, ttol = TOL[precision]
+    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
