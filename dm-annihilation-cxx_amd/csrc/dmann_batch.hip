@@ -119,6 +119,7 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
         if (params[m].precision != prec)
             return bfail(nullptr, DMANN_ERR_ARG, "map %d: all maps of a batch share one precision", m);
         if (params[m].flux_kind < DMANN_FLUX_ANN_NOSOM || params[m].flux_kind > DMANN_FLUX_ANN_ADIABATIC)
+            // (the halo-core plug-in needs a context's core table)
             return bfail(nullptr, DMANN_ERR_ARG, "map %d: unknown flux plug-in %d", m, params[m].flux_kind);
         if (!(params[m].footprint_pad >= 0.0 && params[m].footprint_pad < kPi))
             return bfail(nullptr, DMANN_ERR_ARG, "map %d: footprint_pad outside [0, pi)", m);

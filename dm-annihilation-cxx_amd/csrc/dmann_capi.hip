@@ -43,6 +43,11 @@ struct dmann_ctx {
     int cus = 0;
     bool timing = false;
     std::vector<hipEvent_t> events; // 5 per timed render call
+    // halo cores of the ANN_CORE_NOSOM plug-in (device copies)
+    float4 *d_halos = nullptr;
+    int *d_cell_start = nullptr;
+    int *d_cell_list = nullptr;
+    bool halos_set = false;
     char msg[512] = {0};
 };
 
@@ -108,9 +113,23 @@ static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
     return hipSuccess;
 }
 
+static void free_halos(dmann_ctx *c)
+{
+    (void)hipFree(c->d_halos);
+    (void)hipFree(c->d_cell_start);
+    (void)hipFree(c->d_cell_list);
+    c->d_halos = nullptr;
+    c->d_cell_start = c->d_cell_list = nullptr;
+    c->P.halos = nullptr;
+    c->P.cell_start = c->P.cell_list = nullptr;
+    c->P.nhalos = 0;
+    c->halos_set = false;
+}
+
 static void free_all(dmann_ctx *c)
 {
     free_group(c);
+    free_halos(c);
     hipFree(c->d_parts);
     hipFree(c->d_recs);
     hipFree(c->d_counts);
@@ -157,7 +176,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     *out = nullptr;
     if (p->nside < 1 || p->nside > kMaxNside)
         return fail(nullptr, DMANN_ERR_ARG, "nside %d outside 1..%d", p->nside, kMaxNside);
-    if (p->flux_kind < DMANN_FLUX_ANN_NOSOM || p->flux_kind > DMANN_FLUX_ANN_ADIABATIC)
+    if (p->flux_kind < DMANN_FLUX_ANN_NOSOM || p->flux_kind > DMANN_FLUX_ANN_CORE_NOSOM)
         return fail(nullptr, DMANN_ERR_ARG, "unknown flux plug-in %d", p->flux_kind);
     if (!(p->footprint_pad >= 0.0 && p->footprint_pad < kPi))
         return fail(nullptr, DMANN_ERR_ARG, "footprint_pad %g outside [0, pi)", p->footprint_pad);
@@ -245,6 +264,85 @@ int dmann_set_grouping(dmann_ctx *c, int k, int nside_c)
     return DMANN_OK;
 }
 
+int dmann_set_halo_cores(dmann_ctx *c, const float *halos, int64_t n)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (n < 0 || (n > 0 && !halos) || n > (int64_t)INT32_MAX / 8)
+        return fail(c, DMANN_ERR_ARG, "bad halo table (%lld rows)", (long long)n);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_halos(c);
+    c->halos_set = true;
+    if (n == 0) return DMANN_OK;
+    // rows with the core radius of HaloCore::getCorrection (halocore.cpp:56-59)
+    std::vector<float4> rows((size_t)n);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rmax = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const float *h = halos + 4 * i;
+        const float rc = (float)(h[3] / (0.73 * 40000.0));
+        rows[i] = make_float4(h[0], h[1], h[2], rc);
+        const double ext = fabs((double)rc) * (1 + 1e-6) + 1e-6 * (1 + fabs(h[0]) + fabs(h[1]) + fabs(h[2]));
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fmin(lo[a], (double)h[a] - ext);
+            hi[a] = fmax(hi[a], (double)h[a] + ext);
+        }
+        rmax = fmax(rmax, ext);
+        if (!std::isfinite(h[0]) || !std::isfinite(h[1]) || !std::isfinite(h[2]) || !std::isfinite(rc))
+            return fail(c, DMANN_ERR_ARG, "halo row %lld is not finite", (long long)i);
+    }
+    // cells at least one core diameter wide, at most 128 per axis
+    double span = 0;
+    for (int a = 0; a < 3; a++) span = fmax(span, hi[a] - lo[a]);
+    const double cell = fmax(fmax(2 * rmax, span / 128.0), 1e-30);
+    int dim[3];
+    for (int a = 0; a < 3; a++) dim[a] = (int)fmin(128.0, floor((hi[a] - lo[a]) / cell) + 1);
+    const int64_t ncell = (int64_t)dim[0] * dim[1] * dim[2];
+    std::vector<int> start((size_t)ncell + 1, 0);
+    auto cells_of = [&](int64_t i, int lo_c[3], int hi_c[3]) {
+        const float4 &r = rows[i];
+        const double ext = fabs((double)r.w) * (1 + 1e-6) + 1e-6 * (1 + fabs(r.x) + fabs(r.y) + fabs(r.z));
+        const double cc[3] = {r.x, r.y, r.z};
+        for (int a = 0; a < 3; a++) {
+            lo_c[a] = (int)fmax(0.0, floor((cc[a] - ext - lo[a]) / cell));
+            hi_c[a] = (int)fmin((double)dim[a] - 1, floor((cc[a] + ext - lo[a]) / cell));
+        }
+    };
+    for (int64_t i = 0; i < n; i++) {
+        int a0[3], a1[3];
+        cells_of(i, a0, a1);
+        for (int z = a0[2]; z <= a1[2]; z++)
+            for (int y = a0[1]; y <= a1[1]; y++)
+                for (int x = a0[0]; x <= a1[0]; x++) start[((int64_t)z * dim[1] + y) * dim[0] + x + 1]++;
+    }
+    for (int64_t k = 0; k < ncell; k++) start[k + 1] += start[k];
+    std::vector<int> list((size_t)start[ncell]);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int64_t i = 0; i < n; i++) {
+        int a0[3], a1[3];
+        cells_of(i, a0, a1);
+        for (int z = a0[2]; z <= a1[2]; z++)
+            for (int y = a0[1]; y <= a1[1]; y++)
+                for (int x = a0[0]; x <= a1[0]; x++) list[fill[((int64_t)z * dim[1] + y) * dim[0] + x]++] = (int)i;
+    }
+    HIPCHK(c, hipMalloc(&c->d_halos, n * sizeof(float4)));
+    HIPCHK(c, hipMalloc(&c->d_cell_start, (ncell + 1) * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_cell_list, (list.size() ? list.size() : 1) * sizeof(int)));
+    HIPCHK(c, hipMemcpy(c->d_halos, rows.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_cell_start, start.data(), (ncell + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (!list.empty())
+        HIPCHK(c, hipMemcpy(c->d_cell_list, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->P.halos = c->d_halos;
+    c->P.cell_start = c->d_cell_start;
+    c->P.cell_list = c->d_cell_list;
+    for (int a = 0; a < 3; a++) {
+        c->P.grid_lo[a] = lo[a];
+        c->P.grid_dim[a] = dim[a];
+    }
+    c->P.grid_inv = 1.0 / cell;
+    c->P.nhalos = (int)n;
+    return DMANN_OK;
+}
+
 int dmann_set_stream(dmann_ctx *c, void *stream)
 {
     if (!c) return DMANN_ERR_ARG;
@@ -258,6 +356,8 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
     if (n < 0) return fail(c, DMANN_ERR_ARG, "negative particle count");
     if (n == 0) return DMANN_OK;
     if (!dev_parts) return fail(c, DMANN_ERR_ARG, "null particle pointer");
+    if (c->P.flux_kind == DMANN_FLUX_ANN_CORE_NOSOM && !c->halos_set)
+        return fail(c, DMANN_ERR_ARG, "the halo-core flux plug-in needs dmann_set_halo_cores first");
     if (n > c->max_parts)
         return fail(c, DMANN_ERR_CAPACITY, "%lld particles exceed the context capacity %lld", (long long)n,
                     (long long)c->max_parts);
@@ -283,8 +383,9 @@ int dmann_adjoint_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t 
     if (n < 0) return fail(c, DMANN_ERR_ARG, "negative particle count");
     if (n == 0) return DMANN_OK;
     if (!dev_parts || !dev_grad_map || !dev_grad_parts) return fail(c, DMANN_ERR_ARG, "null pointer");
-    if (c->P.flux_kind == DMANN_FLUX_ANN_ADIABATIC)
-        return fail(c, DMANN_ERR_ARG, "no adjoint for the adiabatic flux plug-in (its factor depends on position)");
+    if (c->P.flux_kind == DMANN_FLUX_ANN_ADIABATIC || c->P.flux_kind == DMANN_FLUX_ANN_CORE_NOSOM)
+        return fail(c, DMANN_ERR_ARG, "no adjoint for the adiabatic or halo-core flux plug-ins (their factor "
+                                      "depends on position)");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, launch_adjoint(reinterpret_cast<const Particle *>(dev_parts), n, c->P, dev_grad_map, dev_grad_parts,
                              c->cfg.medium_blocks, c->stream));
@@ -319,6 +420,8 @@ int dmann_render(dmann_ctx *c, const dmann_particle *host_parts, int64_t n)
     if (n < 0) return fail(c, DMANN_ERR_ARG, "negative particle count");
     if (n == 0) return DMANN_OK;
     if (!host_parts) return fail(c, DMANN_ERR_ARG, "null particle pointer");
+    if (c->P.flux_kind == DMANN_FLUX_ANN_CORE_NOSOM && !c->halos_set)
+        return fail(c, DMANN_ERR_ARG, "the halo-core flux plug-in needs dmann_set_halo_cores first");
     if (n > c->max_parts)
         return fail(c, DMANN_ERR_CAPACITY, "%lld particles exceed the context capacity %lld", (long long)n,
                     (long long)c->max_parts);
@@ -436,6 +539,7 @@ int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)
 int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int32_t *pix, float *w, int64_t cap,
                              double *flux, int *kind)
 {
+    // (the halo-core plug-in's table lives in a context on the device)
     if (!p || !q || p->nside < 1 || p->nside > kMaxNside || p->flux_kind < DMANN_FLUX_ANN_NOSOM ||
         p->flux_kind > DMANN_FLUX_ANN_ADIABATIC)
         return -1;

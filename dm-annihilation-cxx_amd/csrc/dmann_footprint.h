@@ -17,6 +17,45 @@ namespace dmann {
 // both poles.  s0 = sin(theta) as sqrt((1 - cos theta)(1 + cos theta)).
 DMANN_HD bool fast_geometry(double rad, double s0) { return rad <= 0.5 && sin(rad) < s0 * 0.43; }
 
+// coreFunc of src/halocore/halocore.cpp:70-84: float differences, FP64
+// distance; 0.55 (r / radius)^-0.9 inside the core, 1 outside.
+DMANN_HD double core_factor(float x, float y, float z, float hxc, float hyc, float hzc, float radius)
+{
+    const double xx = (x - hxc);
+    const double yy = (y - hyc);
+    const double zz = (z - hzc);
+    const double r = sqrt(xx * xx + yy * yy + zz * zz);
+    double ratio = 1.0;
+    if (r < radius) {
+        if (r > 0) ratio = 0.55 * pow(r / radius, -0.9);
+    }
+    return ratio;
+}
+
+// HaloCore::getCorrection (src/halocore/halocore.cpp:47-68): the product of
+// the factors of all cores in row order; cores that do not contain the
+// particle contribute exactly 1, so only the cores listed in the particle's
+// grid cell are visited (device memory: device code only).
+DMANN_HD double core_correction(const DevParams &P, float x, float y, float z)
+{
+    double c = 1.0;
+    if (P.nhalos <= 0) return c;
+    const double f[3] = {((double)x - P.grid_lo[0]) * P.grid_inv, ((double)y - P.grid_lo[1]) * P.grid_inv,
+                         ((double)z - P.grid_lo[2]) * P.grid_inv};
+    int ic[3];
+    for (int a = 0; a < 3; a++) {
+        if (!(f[a] >= 0.0)) return c;
+        ic[a] = (int)f[a];
+        if (ic[a] >= P.grid_dim[a]) return c;
+    }
+    const int cell = (ic[2] * P.grid_dim[1] + ic[1]) * P.grid_dim[0] + ic[0];
+    for (int k = P.cell_start[cell]; k < P.cell_start[cell + 1]; ++k) {
+        const float4 h = P.halos[P.cell_list[k]];
+        c *= core_factor(x, y, z, h.x, h.y, h.z, h.w);
+    }
+    return c;
+}
+
 // getflux of the map's flux plug-in for particle q at distance d, with the
 // reference's operation order (src/flux/flux_ann_nosom.cpp:12-23,
 // flux_ann_pwave.cpp:13-29, flux_ann_somv.cpp:16-43, flux_ann_somv2.cpp:16-44,
@@ -48,6 +87,10 @@ DMANN_HD double plugin_flux(const DevParams &P, const Particle &q, double d)
     }
     case DMANN_FLUX_DECAY:
         return P.decay_factor * (double)q.mass / (4.0 * kPi * d * d);
+    case DMANN_FLUX_ANN_CORE_NOSOM: {
+        const double r = core_correction(P, q.posx, q.posy, q.posz);
+        return r * r * K * (double)q.dens * (double)q.mass / (4.0 * kPi * d * d);
+    }
     case DMANN_FLUX_ANN_ADIABATIC: {
         // profile factor of the distance to the halo centre, in kpc/h
         const double dx = (double)q.posx - P.cpos[0];

@@ -56,6 +56,15 @@ struct DevParams {
     double decay_factor, vel_to_cgs, c_in_cgs;
     double cpos[3];
     double footprint_pad; // radians added to 2r for pixel membership
+    // halo cores (ANN_CORE_NOSOM): rows (xc, yc, zc, core radius); a uniform
+    // grid of cells, each listing the rows whose core meets it (ascending)
+    const float4 *halos;
+    const int *cell_start;
+    const int *cell_list;
+    double grid_lo[3];
+    double grid_inv; // 1 / cell size
+    int grid_dim[3];
+    int nhalos;
 };
 
 // Device constants of one map from the C-ABI parameters.

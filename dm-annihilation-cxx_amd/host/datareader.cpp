@@ -257,3 +257,22 @@ extern "C" int64_t dmann_read_particles(const char *ini, dmann_particle *out, in
     r.close();
     return bad ? -DMANN_ERR_DOMAIN : n;
 }
+
+// C ABI: halo file rows (xc yc zc radius), read as floats in order as
+// src/halocore/halocore.cpp:17-41 reads them (only complete rows count).
+#include <fstream>
+
+extern "C" int64_t dmann_read_halo_file(const char *path, float *out, int64_t cap)
+{
+    if (!path) return -1;
+    std::ifstream in(path);
+    if (!in.good()) return -1;
+    int64_t n = 0;
+    float v[4];
+    while (in >> v[0] >> v[1] >> v[2] >> v[3]) {
+        if (out && n < cap)
+            for (int k = 0; k < 4; k++) out[4 * n + k] = v[k];
+        n++;
+    }
+    return n;
+}

@@ -10,13 +10,16 @@
 // Accumulation form: DMANN_PRECISION=fp32 | compensated | fp64 in the
 // environment (default compensated, which meets the 1e-5 per-pixel bound
 // against the FP64 CPU generator).  Flux plug-in: DMANN_FLUX=ann_nosom
-// (default) | ann_pwave | ann_somv | ann_somv2 | decay | ann_adiabatic -- the
+// (default) | ann_pwave | ann_somv | ann_somv2 | decay | ann_adiabatic |
+// ann_core_nosom (cores from ./halocore/VL2_Halos.data or DMANN_HALO_FILE) -- the
 // reference links one src/flux/flux_<kind>.cpp per executable
 // (annmap_cuda_nosom, annmap_cuda_pwave, ..., src/Makefile:130-160).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <string>
+#include <vector>
 
 #include "dmann.h"
 #include "mapgenerator.h"
@@ -32,10 +35,11 @@ static int precision_from_env()
 
 static int flux_from_env()
 {
-    static const char *names[] = {"ann_nosom", "ann_pwave", "ann_somv", "ann_somv2", "decay", "ann_adiabatic"};
+    static const char *names[] = {"ann_nosom", "ann_pwave", "ann_somv",      "ann_somv2",
+                                  "decay",     "ann_adiabatic", "ann_core_nosom"};
     const char *s = getenv("DMANN_FLUX");
     if (!s || !*s) return DMANN_FLUX_ANN_NOSOM;
-    for (int k = 0; k < 6; k++)
+    for (int k = 0; k < 7; k++)
         if (!strcmp(s, names[k])) return k;
     printf("Unknown flux plug-in %s\n", s);
     exit(1);
@@ -52,6 +56,24 @@ void MapGenerator::start()
     if (dmann_create(&ctx, &p, cap, 0) != DMANN_OK) {
         printf("HIP initialize error! %s\n", dmann_last_error(nullptr));
         exit(1);
+    }
+    if (p.flux_kind == DMANN_FLUX_ANN_CORE_NOSOM) {
+        // the reference's HaloCore reads ./halocore/VL2_Halos.data
+        // (src/halocore/halocore.h:16); DMANN_HALO_FILE overrides the path
+        const char *hf = getenv("DMANN_HALO_FILE");
+        const std::string path = hf && *hf ? hf : "./halocore/VL2_Halos.data";
+        const int64_t nh = dmann_read_halo_file(path.c_str(), nullptr, 0);
+        if (nh <= 0) {
+            printf("Are you input a empty halo file?\n");
+            exit(0);
+        }
+        std::vector<float> halos((size_t)nh * 4);
+        dmann_read_halo_file(path.c_str(), halos.data(), nh);
+        printf("There are %d halos in consideration!\n", (int)nh);
+        if (dmann_set_halo_cores(ctx, halos.data(), nh) != DMANN_OK) {
+            printf("HIP initialize error! %s\n", dmann_last_error(ctx));
+            exit(1);
+        }
     }
     std::cout << "Creating map!!!" << std::endl;
     std::cout << "---10---20---30---40---50---60---70---80---90--100%\n";

@@ -11,4 +11,4 @@ from . import autograd, autotune, capi, synth  # noqa: F401
 from .autograd import ProjectMap, project_map  # noqa: F401
 from .capi import (FLUX_KINDS, PARTICLE_DTYPE, PREC_COMPENSATED, PREC_FP32, PREC_FP64, Batch, DmannError,  # noqa: F401
                    DmannParams, Projector, load_library, params_for_observer, params_from_ini,
-                   read_fits, read_particles, write_fits, write_ini)
+                   read_fits, read_halo_file, read_particles, write_fits, write_ini)

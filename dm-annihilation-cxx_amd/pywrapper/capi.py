@@ -29,7 +29,7 @@ FLT_MAX = float(np.finfo(np.float32).max)
 
 
 # flux plug-ins (the reference's src/flux/flux_<kind>.cpp), DmannParams.flux_kind
-FLUX_KINDS = ("ann_nosom", "ann_pwave", "ann_somv", "ann_somv2", "decay", "ann_adiabatic")
+FLUX_KINDS = ("ann_nosom", "ann_pwave", "ann_somv", "ann_somv2", "decay", "ann_adiabatic", "ann_core_nosom")
 
 
 class DmannParams(C.Structure):
@@ -76,7 +76,7 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device",
            "dmann_batch_synchronize", "dmann_batch_npix", "dmann_batch_map_device", "dmann_batch_get_map",
            "dmann_batch_get_stats", "dmann_batch_graph_replays", "dmann_batch_destroy",
-           "dmann_batch_last_error", "dmann_read_particles")
+           "dmann_batch_last_error", "dmann_read_particles", "dmann_set_halo_cores", "dmann_read_halo_file")
 
 _lib = None
 
@@ -111,6 +111,9 @@ def load_library():
     L.dmann_npix.restype = i64
     L.dmann_destroy.argtypes = [vp]
     L.dmann_destroy.restype = None
+    L.dmann_set_halo_cores.argtypes = [vp, vp, i64]
+    L.dmann_read_halo_file.argtypes = [C.c_char_p, vp, i64]
+    L.dmann_read_halo_file.restype = i64
     L.dmann_read_particles.argtypes = [C.c_char_p, vp, i64]
     L.dmann_read_particles.restype = i64
     L.dmann_write_healpix_fits.argtypes = [C.c_char_p, vp, i64, C.c_int]
@@ -299,6 +302,12 @@ class Projector:
         _check(self._L.dmann_timing(self._h, int(bool(enable)), ms, C.byref(calls)), self._h)
         return {"prepass": ms[0], "large": ms[1], "sort": ms[2], "medium": ms[3]}, calls.value
 
+    def set_halo_cores(self, halos) -> None:
+        """Halo-core rows (n, 4) float32 (xc, yc, zc, radius) for the
+        ann_core_nosom flux plug-in."""
+        h = np.ascontiguousarray(halos, dtype=np.float32).reshape(-1, 4)
+        _check(self._L.dmann_set_halo_cores(self._h, C.c_void_p(h.ctypes.data), int(h.shape[0])), self._h)
+
     def reset(self) -> None:
         _check(self._L.dmann_reset(self._h), self._h)
 
@@ -323,6 +332,17 @@ class Projector:
             self.close()
         except Exception:
             pass
+
+
+def read_halo_file(path: str) -> np.ndarray:
+    """(n, 4) float32 rows (xc, yc, zc, radius) of a halo-core file."""
+    L = load_library()
+    n = int(L.dmann_read_halo_file(path.encode(), None, 0))
+    if n < 0:
+        raise DmannError(1, f"cannot read {path}")
+    out = np.zeros((n, 4), dtype=np.float32)
+    L.dmann_read_halo_file(path.encode(), C.c_void_p(out.ctypes.data), n)
+    return out
 
 
 def read_particles(ini: str) -> np.ndarray:

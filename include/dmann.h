@@ -105,6 +105,10 @@ typedef struct dmann_params {
  *   ANN_SOMV2      (c / max(v, 5 km/s))^2 / 1e8       (flux_ann_somv2.cpp)
  *   ANN_ADIABATIC  the adiabatic-contraction profile factor squared, of the
  *                  distance to the halo centre        (flux_ann_adiabatic.cpp)
+ *   ANN_CORE_NOSOM the halo-core correction squared: the product over the
+ *                  halos whose core radius contains the particle of
+ *                  0.55 (r / r_core)^-0.9            (flux_ann_core_nosom.cpp,
+ *                  halocore/halocore.cpp:47-84); needs dmann_set_halo_cores
  * and DECAY = decay_factor m / (4 pi d^2)             (flux_decay.cpp). */
 enum dmann_flux {
     DMANN_FLUX_ANN_NOSOM = 0,
@@ -112,7 +116,8 @@ enum dmann_flux {
     DMANN_FLUX_ANN_SOMV = 2,
     DMANN_FLUX_ANN_SOMV2 = 3,
     DMANN_FLUX_DECAY = 4,
-    DMANN_FLUX_ANN_ADIABATIC = 5
+    DMANN_FLUX_ANN_ADIABATIC = 5,
+    DMANN_FLUX_ANN_CORE_NOSOM = 6
 };
 
 /* Counters of the last dmann_get_stats call (cumulative since create/reset). */
@@ -243,6 +248,21 @@ const char *dmann_batch_last_error(const dmann_batch *b);
 int dmann_write_healpix_fits(const char *path, const double *map, int64_t npix, int nside);
 /* Reads such a file; with map == NULL only *nside / *npix are filled. */
 int dmann_read_healpix_fits(const char *path, double *map, int64_t cap, int *nside, int64_t *npix);
+
+/* Halo cores for DMANN_FLUX_ANN_CORE_NOSOM (replaces the reference's
+ * HaloCore, src/halocore/halocore.cpp:17-41): n rows of (xc, yc, zc,
+ * radius) as in its halo file; the core radius is radius / (0.73 * 40000)
+ * in code length units.  Copied to the device with a uniform grid over the
+ * cores, so each particle visits only the cores of its grid cell; the
+ * correction multiplies the containing cores' factors in row order, as the
+ * reference's loop over all rows does.  Replaces any earlier table. */
+int dmann_set_halo_cores(dmann_ctx *ctx, const float *halos, int64_t n);
+
+/* Parses a halo file (whitespace-separated xc yc zc radius rows, read as
+ * floats in order, as src/halocore/halocore.cpp:17-41 reads it) into out
+ * (4 floats per row, up to cap rows); returns the row count (-1 if the file
+ * cannot be read). */
+int64_t dmann_read_halo_file(const char *path, float *out, int64_t cap);
 
 /* Reads the particles the ini file names -- the native file (`datafile`) or
  * a tipsy file with its smooth side files (`partfile`, `densfile`,

@@ -486,6 +486,31 @@ double orc_getflux(double flux_factor, const orc_particle *q, double d)
     return flux_factor * (double)q->dens * (double)q->mass / (4.0 * ORC_PI * d * d);
 }
 
+/* halocore.cpp:70-84: float differences, double distance */
+static double orc_core_func(float x, float y, float z, float hxc, float hyc, float hzc, float radius)
+{
+    double xx = (x - hxc);
+    double yy = (y - hyc);
+    double zz = (z - hzc);
+    double r = sqrt(xx * xx + yy * yy + zz * zz);
+    double ratio = 1.0;
+    if (r < radius) {
+        if (r > 0) ratio = 0.55 * pow(r / radius, -0.9);
+    }
+    return ratio;
+}
+
+double orc_core_correction(const float *halos, int64_t n, float x, float y, float z)
+{
+    double xp = x, yp = y, zp = z;
+    double r = 1.0;
+    for (int64_t i = 0; i < n; i++) {
+        const float *h = halos + 4 * i;
+        r *= orc_core_func((float)xp, (float)yp, (float)zp, h[0], h[1], h[2], (float)(h[3] / (0.73 * 40000.0)));
+    }
+    return r;
+}
+
 double orc_getflux_cfg(const orc_config *c, const orc_particle *q, double d)
 {
     const double K = c->flux_factor;
@@ -517,6 +542,11 @@ double orc_getflux_cfg(const orc_config *c, const orc_particle *q, double d)
     case ORC_FLUX_DECAY:
         /* flux_decay.cpp:10-16 */
         return c->decay_factor * q->mass / (4.0 * ORC_PI * d * d);
+    case ORC_FLUX_ANN_CORE_NOSOM: {
+        /* flux_ann_core_nosom.cpp:10-18: correction squared times nosom */
+        double r = orc_core_correction(c->halos, c->nhalos, q->posx, q->posy, q->posz);
+        return r * r * K * q->dens * q->mass / (4.0 * ORC_PI * d * d);
+    }
     case ORC_FLUX_ANN_ADIABATIC: {
         /* flux_ann_adiabatic.cpp:11-30: profile factor of the distance to
          * the halo centre in kpc/h */

@@ -81,7 +81,8 @@ enum {
     ORC_FLUX_ANN_SOMV = 2,  /* flux_ann_somv.cpp */
     ORC_FLUX_ANN_SOMV2 = 3, /* flux_ann_somv2.cpp */
     ORC_FLUX_DECAY = 4,     /* flux_decay.cpp */
-    ORC_FLUX_ANN_ADIABATIC = 5 /* flux_ann_adiabatic.cpp */
+    ORC_FLUX_ANN_ADIABATIC = 5, /* flux_ann_adiabatic.cpp */
+    ORC_FLUX_ANN_CORE_NOSOM = 6 /* flux_ann_core_nosom.cpp + halocore/halocore.cpp */
 };
 
 typedef struct {
@@ -98,6 +99,8 @@ typedef struct {
     double footprint_pad; /* added to the query_disc radius 2r (0 = genmap.cpp:119;
                              theta_pix/2 = the reference CUDA cutoff,
                              chealpixrender.cu:188) */
+    const float *halos;   /* halo-core rows (xc, yc, zc, radius) for ANN_CORE_NOSOM */
+    int64_t nhalos;
 } orc_config;
 
 typedef struct {
@@ -133,6 +136,9 @@ void orc_config_from_params(const orc_params *p, orc_config *c);
 
 /* ---- angles / flux ---- */
 double orc_getflux(double flux_factor, const orc_particle *q, double d);
+/* HaloCore::getCorrection (halocore.cpp:47-68) over rows (xc, yc, zc,
+ * radius), every row in order. */
+double orc_core_correction(const float *halos, int64_t n, float x, float y, float z);
 /* getflux of the configured plug-in (c->flux_kind). */
 double orc_getflux_cfg(const orc_config *c, const orc_particle *q, double d);
 void orc_calc_angles(double x, double y, double z, double d, const double rot[9],

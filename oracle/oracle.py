@@ -48,14 +48,22 @@ class OrcParams(C.Structure):
 
 
 # flux plug-ins (the reference's src/flux/ files), OrcConfig.flux_kind
-FLUX_KINDS = ("ann_nosom", "ann_pwave", "ann_somv", "ann_somv2", "decay", "ann_adiabatic")
+FLUX_KINDS = ("ann_nosom", "ann_pwave", "ann_somv", "ann_somv2", "decay", "ann_adiabatic", "ann_core_nosom")
 
 
 class OrcConfig(C.Structure):
     _fields_ = [("opos", C.c_double * 3), ("rotmatrix", C.c_double * 9),
                 ("flux_factor", C.c_double), ("mass_cut", C.c_double), ("nside", C.c_int),
                 ("flux_kind", C.c_int), ("decay_factor", C.c_double), ("vel_to_cgs", C.c_double),
-                ("c_in_cgs", C.c_double), ("cpos", C.c_double * 3), ("footprint_pad", C.c_double)]
+                ("c_in_cgs", C.c_double), ("cpos", C.c_double * 3), ("footprint_pad", C.c_double),
+                ("halos", C.c_void_p), ("nhalos", C.c_int64)]
+
+    def set_halos(self, halos):
+        """Attach halo-core rows ((n, 4) float32); the array is kept alive
+        with the config."""
+        self._halos = np.ascontiguousarray(halos, dtype=np.float32).reshape(-1, 4)
+        self.halos = self._halos.ctypes.data
+        self.nhalos = self._halos.shape[0]
 
 
 class OrcStats(C.Structure):
@@ -102,6 +110,8 @@ def lib():
         L.orc_config_from_params.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcConfig)]
         L.orc_getflux.argtypes = [C.c_double, C.c_void_p, C.c_double]
         L.orc_getflux.restype = C.c_double
+        L.orc_core_correction.argtypes = [C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_float]
+        L.orc_core_correction.restype = C.c_double
         L.orc_getflux_cfg.argtypes = [C.POINTER(OrcConfig), C.c_void_p, C.c_double]
         L.orc_getflux_cfg.restype = C.c_double
         L.orc_calc_angles.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double,

@@ -11,6 +11,10 @@ Fixtures:
   flux_kat.json     getflux of every flux plug-in (oracle/_ref/libref_flux_<kind>.so,
                     one per src/flux/flux_<kind>.cpp) on records spanning the
                     velocity saturation points, for two ini files.
+  core_kat.json, VL2_Halos.data
+                    getflux of the halo-core plug-in on records in and around
+                    the cores of the reference's halo file (a copy of
+                    src/halocore/VL2_Halos.data, data of the reference).
   tipsy_kat.npz     the bytes of a small tipsy XDR file (gas, dark, star
                     records), its smooth side files and a mask, and the
                     records DataReader's tipsy path returns for several
@@ -55,7 +59,7 @@ def write_synth_inis():
 
 
 def write_flux_kat(rng):
-    kinds = oracle.FLUX_KINDS
+    kinds = [k for k in oracle.FLUX_KINDS if k != "ann_core_nosom"]  # core_kat.json
     out = {"kinds": list(kinds), "cases": {}}
     for ini in ("VL2_info.ini", "synth_offaxis.ini"):
         path = os.path.join(HERE, ini)
@@ -107,6 +111,44 @@ def tipsy_files(rng, nsph=7, ndark=1234, nstar=5):
     mask = rng.random(ndark) > 0.25
     mbytes = np.array([ndark], dtype="<i4").tobytes() + mask.astype(np.uint8).tobytes()
     return part, side, mbytes
+
+
+def write_core_kat(rng):
+    """getflux of the halo-core plug-in (flux_ann_core_nosom.cpp with
+    halocore.cpp, which reads ./halocore/VL2_Halos.data from the working
+    directory) on records placed inside and around halo cores."""
+    shutil.copyfile(os.path.join(REF, "halocore", "VL2_Halos.data"), os.path.join(HERE, "VL2_Halos.data"))
+    halos = np.loadtxt(os.path.join(HERE, "VL2_Halos.data"), dtype=np.float64)
+    R = oracle.ref_flux("ann_core_nosom")
+    if R is None:
+        sys.exit("oracle/_ref/libref_flux_ann_core_nosom.so is not built")
+    ini = os.path.join(HERE, "VL2_info.ini")
+    n = 64
+    pick = rng.choice(len(halos), n, replace=False)
+    rc = halos[pick, 3] / (0.73 * 40000.0)
+    off = rng.normal(size=(n, 3))
+    off /= np.linalg.norm(off, axis=1, keepdims=True)
+    off *= (rc * rng.uniform(0.0, 1.6, n))[:, None]
+    recs = np.zeros(n, dtype=oracle.PARTICLE_DTYPE)
+    recs["mass"] = rng.uniform(1e-9, 5e3, n)
+    recs["dens"] = rng.uniform(1e-3, 1e6, n)
+    recs["posx"] = halos[pick, 0] + off[:, 0]
+    recs["posy"] = halos[pick, 1] + off[:, 1]
+    recs["posz"] = halos[pick, 2] + off[:, 2]
+    dists = rng.uniform(0.05, 300.0, n)
+    cwd = os.getcwd()
+    os.chdir(REF)
+    try:
+        vals = []
+        for r, d in zip(recs, dists):
+            f32 = np.ascontiguousarray(np.array([r], dtype=oracle.PARTICLE_DTYPE)).view(np.float32).copy()
+            vals.append(float(R.ref_flux(ini.encode(), f32, float(d))).hex())
+    finally:
+        os.chdir(cwd)
+    with open(os.path.join(HERE, "core_kat.json"), "w") as fh:
+        json.dump({"ini": "VL2_info.ini", "halos": "VL2_Halos.data",
+                   "records": recs.view(np.float32).reshape(n, 12).tolist(),
+                   "dists": [float(d).hex() for d in dists], "flux": vals}, fh, indent=1)
 
 
 def write_tipsy_kat(R, rng):
@@ -163,6 +205,7 @@ def main():
         json.dump(out, fh, indent=1)
     write_flux_kat(np.random.default_rng(777))
     write_tipsy_kat(R, np.random.default_rng(4242))
+    write_core_kat(np.random.default_rng(99))
 
     # native reader KAT
     n = 1000

@@ -132,6 +132,8 @@ def test_product_flux_plugins_match_oracle(pw, orc, tmp_path):
     rng = np.random.default_rng(5)
     recs["sigmav"] = np.geomspace(0.05, 300.0, len(recs)) * 1e5 / cfg.vel_to_cgs * rng.uniform(0.9, 1.1, len(recs))
     for k, kind in enumerate(pw.capi.FLUX_KINDS):
+        if kind == "ann_core_nosom":
+            continue  # its core table lives in a device context (GPU tests)
         pk = p.with_flux(kind)
         cfg.flux_kind = k
         for q in recs:

@@ -157,3 +157,26 @@ def test_flux_plugin_closed_forms(orc):
     ratio = f("ann_pwave", s) / f("ann_nosom", s)
     assert ratio == pytest.approx((s * cfg.vel_to_cgs / cfg.c_in_cgs) ** 2, rel=1e-14)
     assert f("decay", s, 3.0) == f("decay", s, 7.0)
+
+
+def test_oracle_core_plugin_matches_reference_kat(orc, pw):
+    """The halo-core plug-in (src/flux/flux_ann_core_nosom.cpp with
+    src/halocore/halocore.cpp, compiled here and run on the reference's halo
+    file) against the oracle, bit for bit; the halo rows are parsed by the
+    library's reader (C++ stream extraction, as HaloCore does)."""
+    with open(os.path.join(GOLD, "core_kat.json")) as fh:
+        kat = json.load(fh)
+    halos = pw.read_halo_file(os.path.join(GOLD, kat["halos"]))
+    assert halos.shape == (20048, 4)
+    assert np.array_equal(halos, np.loadtxt(os.path.join(GOLD, kat["halos"])).astype(np.float32))
+    cfg = orc.config_from_params(orc.params_from_ini(os.path.join(GOLD, kat["ini"])))
+    cfg.flux_kind = orc.FLUX_KINDS.index("ann_core_nosom")
+    cfg.set_halos(halos)
+    recs = np.array(kat["records"], dtype=np.float32).view(orc.PARTICLE_DTYPE).reshape(-1)
+    got = [orc.getflux_cfg(cfg, r, float.fromhex(d)) for r, d in zip(recs, kat["dists"])]
+    want = [float.fromhex(f) for f in kat["flux"]]
+    assert got == want
+    # some records sit inside a core (correction != 1)
+    cfg.flux_kind = orc.FLUX_KINDS.index("ann_nosom")
+    plain = [orc.getflux_cfg(cfg, r, float.fromhex(d)) for r, d in zip(recs, kat["dists"])]
+    assert sum(a != b for a, b in zip(got, plain)) >= 10
