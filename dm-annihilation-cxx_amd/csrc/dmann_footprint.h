@@ -314,10 +314,33 @@ DMANN_HD RingRec ring_setup(const PGeo &g, int nside, int gi)
 // column step is an FP32 quotient and the offset of column kc is its
 // fraction of a column (|.| <= 1/2, from the same FP64 product as kc) times
 // the step, so no FP64 division is made.
+//
+// e0, e1, e2 are the weight terms the ring walks use.  For the 2+2 series
+// form (pv 0) they are the exponent as a quadratic in u = x^2,
+//   log2 w = K4 G(A + B sin^2(x/2)) = e0 + e1 u + e2 u^2 + O(u^3),
+//   G(h) = asin^2(sqrt h) = h + h^2/3 + 8h^3/45 + 4h^4/35 + ...,
+//   e0 = K4 G(A), e1 = K4 G'(A) B/4, e2 = K4 (G''(A) B^2/32 - G'(A) B/48),
+// formed in FP64.  The u^3 remainder is at most (1/360) u^2 of the e1 u
+// term (B in [0, 1], A <= 3.4e-4), below 2e-8 for the pv 0 range
+// u <= 0.0518^2 -- the truncation error of the series form it replaces.
+// For the other forms they are (A, B, 0).
 struct RingLite {
     int cnt, col0, kc_rel, startpix, nr;
     float A, B, offm, step;
+    float e0, e1, e2;
 };
+
+// Exponent polynomial of a pv 0 ring (RingLite).
+DMANN_HD void ring_quad(double A, double B, float K4, float &e0, float &e1, float &e2)
+{
+    const double G0 = A * (1.0 + A * (1.0 / 3.0 + A * (8.0 / 45.0 + A * (4.0 / 35.0))));
+    const double G1 = 1.0 + A * (2.0 / 3.0 + A * (8.0 / 15.0 + A * (16.0 / 35.0)));
+    const double G2 = 2.0 / 3.0 + A * (16.0 / 15.0 + A * (48.0 / 35.0));
+    const double K = (double)K4;
+    e0 = (float)(K * G0);
+    e1 = (float)(K * G1 * B * 0.25);
+    e2 = (float)(K * (G2 * B * B * (1.0 / 32.0) - G1 * B * (1.0 / 48.0)));
+}
 
 DMANN_HD RingLite ring_lite(const PGeo &g, const RingGeo &G, int iz)
 {
@@ -359,6 +382,13 @@ DMANN_HD RingLite ring_lite(const PGeo &g, const RingGeo &G, int iz)
     if (A < 0) A = 0;
     o.A = (float)A;
     o.B = (float)(s * g.s0);
+    if (g.pv == 0) {
+        ring_quad(A, s * g.s0, g.K4, o.e0, o.e1, o.e2);
+    } else {
+        o.e0 = o.A;
+        o.e1 = o.B;
+        o.e2 = 0.0f;
+    }
     o.step = (float)kTwoPi / (float)o.nr;
     o.offm = (float)(((double)kc + shift) - a) * o.step;
     o.kc_rel = kc - o.col0;
@@ -432,10 +462,11 @@ DMANN_HD float weight32(float x, float A, float B, float Ac, float D, float K4)
 }
 
 // The fast form with the series lengths of fast_variant (PV 2 is
-// weight32<0>; PV 0 and 1 drop the terms below 2e-8 relative), with the
-// sin^2 coefficients folded into the ring's B once per ring segment
-// (ring_coef), so a pixel costs 6 VALU operations + exp2 in the 2+2 form:
-//   h = A + u (B/4 - u B/48 [+ u^2 B/1440]),  u = x^2.
+// weight32<0>; PV 1 drops the terms below 2e-8 relative), with the sin^2
+// coefficients folded into the ring's B once per ring segment (ring_coef):
+//   h = A + u (B/4 - u B/48 + u^2 B/1440),  u = x^2.
+// PV 0 evaluates the ring's exponent polynomial (RingLite e0..e2): a pixel
+// costs 4 VALU operations + exp2.
 struct RingCoef {
     float A, c1, c2, c3;
 };
@@ -455,11 +486,24 @@ DMANN_HD float weight_coef(float x, const RingCoef &c, float K4)
         return weight32<0>(x, c.A, c.c1 * 4.0f, 0.0f, 0.0f, K4);
     } else {
         const float u = x * x;
-        const float p = PV == 0 ? fmaf(u, c.c2, c.c1) : fmaf(u, fmaf(u, c.c3, c.c2), c.c1);
+        const float p = fmaf(u, fmaf(u, c.c3, c.c2), c.c1);
         const float h = fmaf(u, p, c.A);
-        float gq = PV == 0 ? K4 * (1.0f / 3.0f) : fmaf(h, K4 * (8.0f / 45.0f), K4 * (1.0f / 3.0f));
+        float gq = fmaf(h, K4 * (8.0f / 45.0f), K4 * (1.0f / 3.0f));
         gq = fmaf(h, gq, K4);
         return fast_exp2(h * gq);
+    }
+}
+
+// Weight of a pixel at offset x on a ring with terms (e0, e1, e2) (RingLite)
+// in series form PV.
+template <int PV>
+DMANN_HD float ring_term_weight(float x, float e0, float e1, float e2, float K4)
+{
+    if constexpr (PV == 0) {
+        const float u = x * x;
+        return fast_exp2(fmaf(u, fmaf(u, e2, e1), e0));
+    } else {
+        return weight_coef<PV>(x, ring_coef(e0, e1), K4);
     }
 }
 
@@ -481,11 +525,10 @@ DMANN_HD float seg_x(const RingLite &rl, int t)
 DMANN_HD float lite_weight32(const PGeo &g, const RingLite &rl, int t)
 {
     const float x = seg_x(rl, t);
-    const RingCoef c = ring_coef(rl.A, rl.B);
     switch (g.pv) {
-    case 0: return weight_coef<0>(x, c, g.K4);
-    case 1: return weight_coef<1>(x, c, g.K4);
-    default: return weight_coef<2>(x, c, g.K4);
+    case 0: return ring_term_weight<0>(x, rl.e0, rl.e1, rl.e2, g.K4);
+    case 1: return ring_term_weight<1>(x, rl.e0, rl.e1, rl.e2, g.K4);
+    default: return ring_term_weight<2>(x, rl.e0, rl.e1, rl.e2, g.K4);
     }
 }
 

@@ -31,13 +31,18 @@ namespace dmann {
 
 // LDS budget of one wave: 20 KB, so 8 single-wave blocks fit a 160 KB CU.
 #ifndef DMANN_GROUP_CAP
-#define DMANN_GROUP_CAP 3760
+#define DMANN_GROUP_CAP 3704
 #endif
 constexpr int kGroupCapF32 = DMANN_GROUP_CAP; // FP32 window slots
+// one-table path for groups whose window fits one band
+#ifndef DMANN_GROUP_ONEBAND
+#define DMANN_GROUP_ONEBAND 1
+#endif
 
 struct GroupLDS {
     // ring table of up to 64 rings of one member
-    float4 rf[64];   // A, B, offm, step
+    float4 rf[64];   // e0, e1, e2, step (RingLite weight terms)
+    float rx[64];    // offm
     int4 ri[64];     // kc_rel, cnt, first segment, end segment
     int4 rs[64];     // window slot offset, window columns, column offset in the window ring, nr
     int2 rd[64];     // col0, first pixel of the ring (direct deposits)
@@ -60,25 +65,27 @@ __device__ __forceinline__ double wave_max_d(double v) { return wave_reduce_full
 __device__ __forceinline__ int wave_min_i(int v) { return wave_reduce_full<DppMin>(v, 0x7fffffff); }
 __device__ __forceinline__ int wave_max_i(int v) { return wave_reduce_full<DppMax>(v, -0x7fffffff - 1); }
 
-// Ring terms packed for the per-wave cache: (A, B, offm, step) and
-// (cnt, col0, kc_rel in the low / nr in the high 16 bits, startpix).
+// Ring terms packed for the per-wave cache: (e0, e1, e2, offm) and
+// (cnt, col0, kc_rel in the low / nr in the high 16 bits, startpix); the
+// step is formed again from nr (the same correctly rounded FP32 quotient).
 __device__ __forceinline__ void ring_pack(const RingLite &rl, float4 &f, int4 &i)
 {
-    f = make_float4(rl.A, rl.B, rl.offm, rl.step);
+    f = make_float4(rl.e0, rl.e1, rl.e2, rl.offm);
     i = make_int4(rl.cnt, rl.col0, (int)(((unsigned)rl.nr << 16) | ((unsigned)rl.kc_rel & 0xffffu)), rl.startpix);
 }
 __device__ __forceinline__ RingLite ring_unpack(const float4 &f, const int4 &i)
 {
     RingLite rl;
-    rl.A = f.x;
-    rl.B = f.y;
-    rl.offm = f.z;
-    rl.step = f.w;
+    rl.e0 = f.x;
+    rl.e1 = f.y;
+    rl.e2 = f.z;
+    rl.offm = f.w;
     rl.cnt = i.x;
     rl.col0 = i.y;
     rl.nr = (int)((unsigned)i.z >> 16);
     rl.kc_rel = (int)(short)(i.z & 0xffff);
     rl.startpix = i.w;
+    rl.step = (float)kTwoPi / (float)rl.nr;
     return rl;
 }
 
@@ -110,7 +117,8 @@ __device__ __forceinline__ int ring_table_fill(const RingLite &rl, bool on, int 
         for (int l = llo; l < lhi; ++l) S.q0[l] = (unsigned char)lane;
     }
     if (on) {
-        S.rf[lane] = make_float4(rl.A, rl.B, rl.offm, rl.step);
+        S.rf[lane] = make_float4(rl.e0, rl.e1, rl.e2, rl.step);
+        S.rx[lane] = rl.offm;
         S.ri[lane] = make_int4(rl.kc_rel, cnt, incl - nseg, incl);
         S.rd[lane] = make_int2(rl.col0, rl.startpix);
         if (band_ring0 >= 0) {
@@ -161,15 +169,15 @@ __device__ __forceinline__ int ring_table(const PGeo &g, const RingGeo &G, int i
 // Deposits pixels t0 .. t0+len-1 of table ring q straight into the map
 // (global atomics), for bands whose window does not fit the LDS buffer.
 template <int ACC, int PV>
-__device__ __forceinline__ void deposit_direct(const GroupLDS &S, int q, int t0, int len, float x0, float step,
-                                               float A, float B, float K4, float sc, const MapRef &map)
+__device__ __forceinline__ void deposit_direct(const GroupLDS &S, int q, int t0, int len, float x0, const float4 &f,
+                                               float K4, float sc, const MapRef &map)
 {
     const int2 d = S.rd[q];
     const int nr = S.rs[q].w;
 #pragma unroll
     for (int t = 0; t < kSeg; ++t) {
         if (t < len) {
-            const float v = weight_coef<PV>(fmaf((float)t, step, x0), ring_coef(A, B), K4) * sc;
+            const float v = ring_term_weight<PV>(fmaf((float)t, f.w, x0), f.x, f.y, f.z, K4) * sc;
             int col = d.x + t0 + t;
             if (col < 0) col += nr;
             else if (col >= nr) col -= nr;
@@ -178,10 +186,20 @@ __device__ __forceinline__ void deposit_direct(const GroupLDS &S, int q, int t0,
     }
 }
 
+// 1 for pixel t of a segment of len pixels (t < len), else 0: one VALU
+// operation (a subtraction with the output clamp), since len and t are
+// integers.
+__device__ __forceinline__ float seg_mask(float lenf, int t)
+{
+    return __builtin_amdgcn_fmed3f(lenf - (float)t, 0.0f, 1.0f);
+}
+
 // Walks the table's segments: lane L takes a contiguous run of segments
 // (NS/64 each), kSeg pixels per segment with the ring terms loaded once per
 // segment.  Every lane evaluates all kSeg pixels of its segment; those past
-// the ring's end are masked to zero, so the loop has no divergent branch.
+// the ring's end are masked to zero (a factor 0 in an fma: the weights
+// there are finite), so the loop has no divergent branch.  Pixel 0 of a
+// segment always lies on the ring.
 // MODE 0 sums the weights (returns the lane's sum).  MODE 1 deposits
 // weight * sc: into the LDS window (WINDOW), where each band ring owns
 // window-columns + kSeg slots, so the masked tail of a ring's last segment
@@ -204,13 +222,14 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
             const int4 r = S.ri[q];
             const int t0 = (s - r.z) * kSeg;
             const int len = r.y - t0 < kSeg ? r.y - t0 : kSeg;
-            const float x0 = fmaf((float)(t0 - r.x), f.w, f.z);
-            const RingCoef rc = ring_coef(f.x, f.y);
+            const float lenf = (float)len;
+            const float x0 = fmaf((float)(t0 - r.x), f.w, S.rx[q]);
             if constexpr (MODE == 0) {
 #pragma unroll
                 for (int t = 0; t < kSeg; ++t) {
-                    const float w = weight_coef<PV>(fmaf((float)t, f.w, x0), rc, K4);
-                    part += t < len ? w : 0.0f;
+                    const float x = t == 0 ? x0 : fmaf((float)t, f.w, x0);
+                    const float w = ring_term_weight<PV>(x, f.x, f.y, f.z, K4);
+                    part = t == 0 ? part + w : fmaf(w, seg_mask(lenf, t), part);
                 }
             } else if constexpr (WINDOW) {
                 const int4 w = S.rs[q];
@@ -220,14 +239,15 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
                     float *a = &S.acc[w.x + off];
 #pragma unroll
                     for (int t = 0; t < kSeg; ++t) {
-                        const float v = weight_coef<PV>(fmaf((float)t, f.w, x0), rc, K4) * sc;
-                        a[t] += t < len ? v : 0.0f;
+                        const float x = t == 0 ? x0 : fmaf((float)t, f.w, x0);
+                        const float v = ring_term_weight<PV>(x, f.x, f.y, f.z, K4);
+                        a[t] = fmaf(v, t == 0 ? sc : sc * seg_mask(lenf, t), a[t]);
                     }
                 } else {
-                    deposit_direct<ACC, PV>(S, q, t0, len, x0, f.w, f.x, f.y, K4, sc, map);
+                    deposit_direct<ACC, PV>(S, q, t0, len, x0, f, K4, sc, map);
                 }
             } else {
-                deposit_direct<ACC, PV>(S, q, t0, len, x0, f.w, f.x, f.y, K4, sc, map);
+                deposit_direct<ACC, PV>(S, q, t0, len, x0, f, K4, sc, map);
             }
         }
         if ((i & 3) == 3) {
@@ -303,6 +323,85 @@ __device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const 
     }
 }
 
+// A band of window rings b0 .. b0+BH-1 (BH <= 64, rings up to R1) whose
+// slots fit the LDS buffer: band table, block owners and zeroed slots.
+// The window spans longitudes phic + [wlo, whi] (whole rings when full).
+// direct: ring b0 alone exceeds the buffer; its deposits go to global
+// atomics and the table only holds the ring layout.
+struct Band {
+    int BH;      // rings in the band
+    int Sb;      // window slots in use
+    bool direct;
+};
+__device__ __forceinline__ Band band_setup(GroupLDS &S, int lane, int nside, int b0, int R1, double phic,
+                                           double wlo, double whi, bool full)
+{
+    const int ring = b0 + lane;
+    const bool vr = ring <= R1;
+    int startpix = 0, nr = 1, shifted = 0, cl = 0, ncols = 0;
+    if (vr) {
+        ring_info(nside, ring, startpix, nr, shifted);
+        const double shift = shifted ? 0.5 : 0.0;
+        if (full) {
+            cl = 0;
+            ncols = nr;
+        } else {
+            cl = (int)floor((double)nr * kInvTwoPi * (phic + wlo) - shift) - 1;
+            const int ch = (int)floor((double)nr * kInvTwoPi * (phic + whi) - shift) + 1;
+            ncols = ch - cl + 1 < nr ? ch - cl + 1 : nr;
+        }
+    }
+    const int alloc = vr ? (ncols + 2 * kSeg - 1) / kSeg * kSeg : 0;
+    const int incl = wave_scan_full<DppAdd>(alloc, 0);
+    const bool fits = vr && incl <= kGroupCapF32;
+    Band bd;
+    bd.BH = __popcll(__ballot(fits));
+    bd.direct = bd.BH == 0;
+    bd.Sb = 0;
+    if (bd.direct) {
+        bd.BH = (R1 - b0 + 1) < 64 ? (R1 - b0 + 1) : 64;
+        // direct deposits still need the band table for slot terms
+        if (lane < bd.BH) {
+            S.bt[lane] = make_int4(0, 0, nr, startpix);
+            S.bnc[lane] = 0;
+        }
+    } else {
+        bd.Sb = wave_read(incl, bd.BH - 1);
+        if (lane < bd.BH) {
+            int clw = cl % nr;
+            if (clw < 0) clw += nr;
+            S.bt[lane] = make_int4(incl - alloc, clw, nr, startpix);
+            S.bnc[lane] = ncols;
+            for (int k = (incl - alloc) / kSeg; k < incl / kSeg; ++k) S.ros[k] = (unsigned char)lane;
+        }
+        if (lane == 0) S.bt[bd.BH] = make_int4(bd.Sb, 0, 1, 0);
+        for (int t = lane; t < bd.Sb; t += 64) S.acc[t] = 0.0f;
+    }
+    wave_sync();
+    return bd;
+}
+
+// Longitude window of the members with on set: the lead member's phi and
+// the extent of the others' ranges relative to it.
+struct Window {
+    double phic, wlo, whi;
+    bool full;
+};
+__device__ __forceinline__ Window group_window(bool on, double phi, double dphi)
+{
+    const uint64_t onmask = __ballot(on);
+    const int lead = __ffsll((unsigned long long)onmask) - 1;
+    Window w;
+    w.phic = wave_read(phi, lead);
+    double dl = phi - w.phic;
+    if (dl > kPi) dl -= kTwoPi;
+    if (dl < -kPi) dl += kTwoPi;
+    w.wlo = wave_min_d(on ? dl - dphi : 1e30);
+    w.whi = wave_max_d(on ? dl + dphi : -1e30);
+    w.full = (w.whi - w.wlo) >= kTwoPi - 1e-9;
+    return w;
+}
+
 #ifdef DMANN_GROUP_WPE
 #define DMANN_GROUP_ATTR __attribute__((amdgpu_waves_per_eu(DMANN_GROUP_WPE)))
 #else
@@ -316,7 +415,6 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
                                                    float4 *__restrict__ tab_f, int4 *__restrict__ tab_i)
 {
     __shared__ GroupLDS S;
-    constexpr int CAP = kGroupCapF32;
     const int lane = threadIdx.x;
     const RingGeo G = ring_geo(nside);
     const unsigned npix = (unsigned)npix_of(nside);
@@ -353,6 +451,52 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
             my_flux = mg.flux;
             my_p0 = mg.p0;
         }
+#if DMANN_GROUP_ONEBAND
+        // ---- a group whose whole window fits one band: each member is
+        // normalised and deposited in turn from one ring table (members
+        // span at most 64 rings then), with no cached ring terms
+        {
+            const bool gon = lane < nm && my_hi >= my_lo;
+            const int G0 = wave_min_i(gon ? my_lo : 0x7fffffff);
+            const int G1 = wave_max_i(gon ? my_hi : -0x7fffffff);
+            if (G0 <= G1 && G1 - G0 < 64) {
+                const Window gw = group_window(gon, my_phi, my_dphi);
+                const Band bd = band_setup(S, lane, nside, G0, G1, gw.phic, gw.wlo, gw.whi, gw.full);
+                if (!bd.direct && bd.BH > G1 - G0) {
+                    for (int j = 0; j < nm; ++j) {
+                        PGeo g;
+                        g.phi = wave_read(my_phi, j);
+                        g.z0 = wave_read(my_z0, j);
+                        g.s0 = wave_read(my_s0, j);
+                        g.xa = wave_read(my_xa, j);
+                        g.cosr = wave_read(my_cosr, j);
+                        g.K4 = wave_read(my_K4, j);
+                        g.pv = wave_read(my_pv, j);
+                        g.irmin = wave_read(my_lo, j);
+                        g.irmax = wave_read(my_hi, j);
+                        const int n = g.irmax >= g.irmin ? g.irmax - g.irmin + 1 : 0;
+                        const int NS = ring_table(g, G, g.irmin, n, S, lane, G0);
+                        const int T = wave_reduce_full<DppAdd>(lane < n ? S.ri[lane].y : 0, 0);
+                        const double norm =
+                            wave_reduce_full<DppAdd>(walk_pv<0, ACC, false>(g.pv, S, NS, lane, g.K4, 0.0f, map), 0.0);
+                        if (T < 2) {
+                            if (lane == j) acc_point<ACC>(map, my_p0, my_flux);
+                            n_fb++;
+                        } else {
+                            n_ext++;
+                            n_contrib += (unsigned long long)T;
+                            const float sc = (float)(wave_read(my_flux, j) / norm);
+                            walk_pv<1, ACC, true>(g.pv, S, NS, lane, g.K4, sc, map);
+                        }
+                        wave_sync();
+                    }
+                    flush_band<ACC>(S, bd.Sb, lane, map, npix);
+                    wave_sync();
+                    continue;
+                }
+            }
+        }
+#endif
         for (int j = 0; j < nm; ++j) {
             PGeo g;
             g.phi = wave_read(my_phi, j);
@@ -398,58 +542,12 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
         const int R0 = wave_min_i(my_on ? my_lo : 0x7fffffff);
         const int R1 = wave_max_i(my_on ? my_hi : -0x7fffffff);
         if (R0 > R1) continue;
-        const uint64_t onmask = __ballot(my_on);
-        const int lead = __ffsll((unsigned long long)onmask) - 1;
-        const double phic = wave_read(my_phi, lead);
-        double dl = my_phi - phic;
-        if (dl > kPi) dl -= kTwoPi;
-        if (dl < -kPi) dl += kTwoPi;
-        const double wlo = wave_min_d(my_on ? dl - my_dphi : 1e30);
-        const double whi = wave_max_d(my_on ? dl + my_dphi : -1e30);
-        const bool full = (whi - wlo) >= kTwoPi - 1e-9;
+        const Window win = group_window(my_on, my_phi, my_dphi);
         // ---- phase 2: bands of rings that fit the LDS buffer
         for (int b0 = R0; b0 <= R1;) {
-            const int ring = b0 + lane;
-            const bool vr = ring <= R1;
-            int startpix = 0, nr = 1, shifted = 0, cl = 0, ncols = 0;
-            if (vr) {
-                ring_info(nside, ring, startpix, nr, shifted);
-                const double shift = shifted ? 0.5 : 0.0;
-                if (full) {
-                    cl = 0;
-                    ncols = nr;
-                } else {
-                    cl = (int)floor((double)nr * kInvTwoPi * (phic + wlo) - shift) - 1;
-                    const int ch = (int)floor((double)nr * kInvTwoPi * (phic + whi) - shift) + 1;
-                    ncols = ch - cl + 1 < nr ? ch - cl + 1 : nr;
-                }
-            }
-            const int alloc = vr ? (ncols + 2 * kSeg - 1) / kSeg * kSeg : 0;
-            const int incl = wave_scan_full<DppAdd>(alloc, 0);
-            const bool fits = vr && incl <= CAP;
-            int BH = __popcll(__ballot(fits));
-            const bool direct = BH == 0; // one window ring exceeds the buffer
-            if (direct) BH = (R1 - b0 + 1) < 64 ? (R1 - b0 + 1) : 64;
-            int Sb = 0;
-            if (!direct) {
-                Sb = wave_read(incl, BH - 1);
-                if (lane < BH) {
-                    int clw = cl % nr;
-                    if (clw < 0) clw += nr;
-                    S.bt[lane] = make_int4(incl - alloc, clw, nr, startpix);
-                    S.bnc[lane] = ncols;
-                    for (int k = (incl - alloc) / kSeg; k < incl / kSeg; ++k) S.ros[k] = (unsigned char)lane;
-                }
-                if (lane == 0) S.bt[BH] = make_int4(Sb, 0, 1, 0);
-                for (int t = lane; t < Sb; t += 64) S.acc[t] = 0.0f;
-            } else {
-                // direct deposits still need the band table for slot terms
-                if (lane < BH) {
-                    S.bt[lane] = make_int4(0, 0, nr, startpix);
-                    S.bnc[lane] = 0;
-                }
-            }
-            wave_sync();
+            const Band bd = band_setup(S, lane, nside, b0, R1, win.phic, win.wlo, win.whi, win.full);
+            const int BH = bd.BH, Sb = bd.Sb;
+            const bool direct = bd.direct;
             const int b1 = b0 + BH - 1;
             // members meeting the band, in order; the cached ring terms of
             // the next one are loaded while the current one is walked
@@ -495,6 +593,8 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
                         g.s0 = wave_read(my_s0, j);
                         g.xa = wave_read(my_xa, j);
                         g.cosr = wave_read(my_cosr, j);
+                        g.K4 = wave_read(my_K4, j);
+                        g.pv = wave_read(my_pv, j);
                         rl = ring_lite(g, G, rlo + lane);
                     }
                 }
