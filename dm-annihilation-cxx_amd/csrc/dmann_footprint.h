@@ -319,10 +319,10 @@ DMANN_HD RingRec ring_setup(const PGeo &g, int nside, int gi)
 // form (pv 0) they are the exponent as a quadratic in u = x^2,
 //   log2 w = K4 G(A + B sin^2(x/2)) = e0 + e1 u + e2 u^2 + O(u^3),
 //   G(h) = asin^2(sqrt h) = h + h^2/3 + 8h^3/45 + 4h^4/35 + ...,
-//   e0 = K4 G(A), e1 = K4 G'(A) B/4, e2 = K4 (G''(A) B^2/32 - G'(A) B/48),
-// formed in FP64.  The u^3 remainder is at most (1/360) u^2 of the e1 u
+//   e0 = K4 G(A), e1 = K4 G'(A) B/4, e2 = K4 (G''(A) B^2/32 - G'(A) B/48).  The u^3 remainder is at most (1/360) u^2 of the e1 u
 // term (B in [0, 1], A <= 3.4e-4), below 2e-8 for the pv 0 range
-// u <= 0.0518^2 -- the truncation error of the series form it replaces.
+// u <= 0.0518^2 -- the truncation error of the series form it replaces
+// (ring_quad).
 // For the other forms they are (A, B, 0).
 struct RingLite {
     int cnt, col0, kc_rel, startpix, nr;
@@ -330,16 +330,18 @@ struct RingLite {
     float e0, e1, e2;
 };
 
-// Exponent polynomial of a pv 0 ring (RingLite).
-DMANN_HD void ring_quad(double A, double B, float K4, float &e0, float &e1, float &e2)
+// Exponent polynomial of a pv 0 ring (RingLite), in FP32 from the FP32
+// terms A, B: A <= 3.4e-4, so the series of G, G', G'' are exact to FP32
+// rounding with three terms, and the rounding of A and B (as in the series
+// form) dominates the error.
+DMANN_HD void ring_quad(float A, float B, float K4, float &e0, float &e1, float &e2)
 {
-    const double G0 = A * (1.0 + A * (1.0 / 3.0 + A * (8.0 / 45.0 + A * (4.0 / 35.0))));
-    const double G1 = 1.0 + A * (2.0 / 3.0 + A * (8.0 / 15.0 + A * (16.0 / 35.0)));
-    const double G2 = 2.0 / 3.0 + A * (16.0 / 15.0 + A * (48.0 / 35.0));
-    const double K = (double)K4;
-    e0 = (float)(K * G0);
-    e1 = (float)(K * G1 * B * 0.25);
-    e2 = (float)(K * (G2 * B * B * (1.0 / 32.0) - G1 * B * (1.0 / 48.0)));
+    const float G0 = A * fmaf(A, fmaf(A, 8.0f / 45.0f, 1.0f / 3.0f), 1.0f);
+    const float G1 = fmaf(A, fmaf(A, 8.0f / 15.0f, 2.0f / 3.0f), 1.0f);
+    const float G2 = fmaf(A, 16.0f / 15.0f, 2.0f / 3.0f);
+    e0 = K4 * G0;
+    e1 = (K4 * G1) * (B * 0.25f);
+    e2 = K4 * (B * fmaf(G2 * (1.0f / 32.0f), B, G1 * (-1.0f / 48.0f)));
 }
 
 DMANN_HD RingLite ring_lite(const PGeo &g, const RingGeo &G, int iz)
@@ -383,7 +385,7 @@ DMANN_HD RingLite ring_lite(const PGeo &g, const RingGeo &G, int iz)
     o.A = (float)A;
     o.B = (float)(s * g.s0);
     if (g.pv == 0) {
-        ring_quad(A, s * g.s0, g.K4, o.e0, o.e1, o.e2);
+        ring_quad(o.A, o.B, g.K4, o.e0, o.e1, o.e2);
     } else {
         o.e0 = o.A;
         o.e1 = o.B;

@@ -66,12 +66,13 @@ __device__ __forceinline__ int wave_min_i(int v) { return wave_reduce_full<DppMi
 __device__ __forceinline__ int wave_max_i(int v) { return wave_reduce_full<DppMax>(v, -0x7fffffff - 1); }
 
 // Ring terms packed for the per-wave cache: (e0, e1, e2, offm) and
-// (cnt, col0, kc_rel in the low / nr in the high 16 bits, startpix); the
-// step is formed again from nr (the same correctly rounded FP32 quotient).
+// (cnt, col0, kc_rel in the low / nr in the high 16 bits, step).  The
+// ring's first pixel is not kept: band tables hold it.
 __device__ __forceinline__ void ring_pack(const RingLite &rl, float4 &f, int4 &i)
 {
     f = make_float4(rl.e0, rl.e1, rl.e2, rl.offm);
-    i = make_int4(rl.cnt, rl.col0, (int)(((unsigned)rl.nr << 16) | ((unsigned)rl.kc_rel & 0xffffu)), rl.startpix);
+    i = make_int4(rl.cnt, rl.col0, (int)(((unsigned)rl.nr << 16) | ((unsigned)rl.kc_rel & 0xffffu)),
+                  __float_as_int(rl.step));
 }
 __device__ __forceinline__ RingLite ring_unpack(const float4 &f, const int4 &i)
 {
@@ -84,8 +85,8 @@ __device__ __forceinline__ RingLite ring_unpack(const float4 &f, const int4 &i)
     rl.col0 = i.y;
     rl.nr = (int)((unsigned)i.z >> 16);
     rl.kc_rel = (int)(short)(i.z & 0xffff);
-    rl.startpix = i.w;
-    rl.step = (float)kTwoPi / (float)rl.nr;
+    rl.step = __int_as_float(i.w);
+    rl.startpix = -1;
     return rl;
 }
 
@@ -112,7 +113,8 @@ __device__ __forceinline__ int ring_table_fill(const RingLite &rl, bool on, int 
     const int NS = wave_read(incl, 63);
     // lanes whose first segment (lane NS / 64) lies in this ring's range
     if (nseg > 0) {
-        const float rns = 1.0f / (float)NS;
+        // (ceil_div_small corrects the approximate reciprocal's quotient)
+        const float rns = __builtin_amdgcn_rcpf((float)NS);
         const int llo = ceil_div_small((incl - nseg) * 64, NS, rns), lhi = ceil_div_small(incl * 64, NS, rns);
         for (int l = llo; l < lhi; ++l) S.q0[l] = (unsigned char)lane;
     }
@@ -120,7 +122,7 @@ __device__ __forceinline__ int ring_table_fill(const RingLite &rl, bool on, int 
         S.rf[lane] = make_float4(rl.e0, rl.e1, rl.e2, rl.step);
         S.rx[lane] = rl.offm;
         S.ri[lane] = make_int4(rl.kc_rel, cnt, incl - nseg, incl);
-        S.rd[lane] = make_int2(rl.col0, rl.startpix);
+        if (band_ring0 < 0) S.rd[lane] = make_int2(rl.col0, rl.startpix);
         if (band_ring0 >= 0) {
             // slot terms: the band ring owns ncols + kSeg slots from boff;
             // a segment may fill slots off .. off+len-1 below the limit.
@@ -129,6 +131,7 @@ __device__ __forceinline__ int ring_table_fill(const RingLite &rl, bool on, int 
             // the last segment off the first one (else limit 0: direct).
             const int rel = iz0 + lane - band_ring0;
             const int4 b = S.bt[rel];
+            S.rd[lane] = make_int2(rl.col0, b.w);
             // col0 in [-nr/2 - 1, nr), first window column in [0, nr)
             int off = rl.col0 - b.y;
             if (off < 0) off += rl.nr;
