@@ -1,8 +1,9 @@
 // Grouped footprint rendering with wave-private LDS accumulation.
 //
 // Global FP32 atomics on MI355X retire about one dword per clock per L2
-// channel (~2.6e11/s chip-wide) whatever the locality, and LDS atomics are
-// no faster; with ~1e12 footprint pixels per map, one atomic per pixel is
+// channel (~2.6e11/s chip-wide) even when an instruction's 64 lanes hit
+// consecutive dwords (fewer when they spread over more cache lines), and
+// LDS atomics are no faster; with ~1e12 footprint pixels per map, one atomic per pixel is
 // the bottleneck.  Here the medium-class particles are first sorted by a
 // coarse sky cell (counting sort), then one wavefront takes K consecutive
 // particles (a group of neighbours):
