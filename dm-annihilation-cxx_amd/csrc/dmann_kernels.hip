@@ -24,6 +24,9 @@
 
 namespace dmann {
 
+// size classes of the sky-covering kernel's largest-first order
+constexpr int kLargeClasses = 10;
+
 // ---------------------------------------------------------------- pre-pass
 
 // One thread per particle (grid-stride).  Mirrors src/genmap.cpp:82-110 with
@@ -279,84 +282,125 @@ __global__ __launch_bounds__(256) void k_disc_medium(const DiscRec *__restrict__
 // walks whole rings (ring-major) from 64-ring windows, FP64 normalisation.
 template <int ACC>
 __global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ recs, int64_t n,
-                                                    const unsigned *__restrict__ counts, int nside, double pad,
+                                                    unsigned *__restrict__ counts, int nside, double pad,
                                                     MapRef map, unsigned long long *__restrict__ stats)
 {
     __shared__ WinLDS lds[4];
     __shared__ double s_red[4];
     __shared__ int s_cnt[4];
+    __shared__ unsigned s_g, s_list[256];
+    __shared__ int s_wn[4];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     WinLDS &L = lds[wid];
     const unsigned count = counts[1];
-    for (unsigned it = blockIdx.x; it < count; it += gridDim.x) {
-        const PGeo g = particle_geo(recs[n - 1 - (int64_t)it], nside, pad);
-        const int nwin = (g.nrings + 63) / 64;
-        const double vx = sin(g.theta) * cos(g.phi), vy = sin(g.theta) * sin(g.phi);
-        double scale64 = 0.0;
-        for (int pass = 0; pass < 2; ++pass) {
-            double nacc = 0.0;
-            int T = 0;
-            for (int win = wid; win < nwin; win += 4) {
-                const int Tw = setup_window<ACC == kAccF64>(g, nside, win, L, lane, 0);
-                T += Tw;
-                for (int j = 0; j < 64; ++j) {
-                    const int pre_start = j ? L.pre_end[j - 1] : 0;
-                    const int cnt = L.pre_end[j] - pre_start;
-                    if (cnt <= 0) continue;
-                    const float4 f = L.f[j];
-                    const float4 gq = L.gq[j];
-                    const int kref = L.kref[j];
-                    const int4 a = L.a[j];
-                    const double4 dd = L.d[j];
-                    float part = 0.0f;
-                    for (int t = lane; t < cnt; t += 64) {
-                        const int k = pre_start + t;
-                        if constexpr (ACC == kAccF64) {
-                            const double w = weight64(dd.x, dd.y, ((double)(k - a.w) + dd.z) * dd.w, g, vx, vy);
-                            if (pass == 0) nacc += w;
-                            else acc_add<ACC>(map, pix_addr(a, k), 0.0f, (w * scale64));
-                        } else {
-                            const float kf = (float)(k - kref);
-                            float w;
-                            if (g.mode == 0) w = weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, 0.0f, 0.0f, g.K4);
-                            else w = weight32<1>(fmaf(kf, f.w, f.z), f.x, gq.x, gq.y, gq.z, g.K4);
-                            if (pass == 0) part += w;
-                            else acc_add<ACC>(map, pix_addr(a, k), w * (float)scale64, 0.0);
-                        }
-                    }
-                    if constexpr (ACC != kAccF64) nacc += (double)part;
-                }
-                wave_sync();
-            }
-            if (pass == 0) {
-                const double ws = wave_sum(nacc);
-                if (lane == 0) {
-                    s_red[wid] = ws;
-                    s_cnt[wid] = T;
-                }
-                __syncthreads();
-                const double norm = s_red[0] + s_red[1] + s_red[2] + s_red[3];
-                const int Ttot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-                __syncthreads();
-                if (Ttot < 2) {
-                    if (threadIdx.x == 0) {
-                        acc_point<ACC>(map, g.p0, g.flux);
-                        atomicAdd(&stats[kStatFallback], 1ull);
-                    }
-                    break;
-                }
-                if (ACC == kAccF64) scale64 = g.flux / norm;
-                else scale64 = g.flux / norm;
-                if (threadIdx.x == 0) {
-                    atomicAdd(&stats[kStatExt], 1ull);
-                    atomicAdd(&stats[kStatLarge], 1ull);
-                    atomicAdd(&stats[kStatContrib], (unsigned long long)Ttot);
-                    atomicAdd(&stats[kStatLargeContrib], (unsigned long long)Ttot);
-                }
-            }
-        }
+    // Footprints here range from a few thousand pixels to the whole sky, so
+    // blocks take work from a counter (counts[2], zeroed with the class
+    // counters), largest footprints first: work item g is chunk g % nchunk
+    // (256 particles) of size class g / nchunk (floor(log2 est) below the
+    // map's, the last class taking the rest).  Each block compacts the
+    // chunk's particles of that class and renders them in turn.
+    const unsigned nchunk = (count + 255u) / 256u;
+    const int lgmax = 63 - __clzll((unsigned long long)npix_of(nside));
+    for (;;) {
+        if (threadIdx.x == 0) s_g = atomicAdd(&counts[2], 1u);
         __syncthreads();
+        const unsigned g = s_g;
+        if (g >= nchunk * (unsigned)kLargeClasses) break;
+        const int want = (int)(g / nchunk);
+        const unsigned item = (g % nchunk) * 256u + threadIdx.x;
+        bool mine = false;
+        if (item < count) {
+            const int est = recs[n - 1 - (int64_t)item].est;
+            const int lg = est > 0 ? 31 - __clz(est) : 0;
+            int c = lgmax - lg;
+            c = c < 0 ? 0 : (c >= kLargeClasses ? kLargeClasses - 1 : c);
+            mine = c == want;
+        }
+        const uint64_t m = __ballot(mine);
+        if (lane == 0) s_wn[wid] = __popcll(m);
+        __syncthreads();
+        int base = 0;
+        for (int w = 0; w < wid; ++w) base += s_wn[w];
+        if (mine) s_list[base + __popcll(m & ((1ull << lane) - 1ull))] = item;
+        const int nmine = s_wn[0] + s_wn[1] + s_wn[2] + s_wn[3];
+        __syncthreads();
+        for (int l = 0; l < nmine; ++l) {
+            const unsigned it = s_list[l];
+            const PGeo g = particle_geo(recs[n - 1 - (int64_t)it], nside, pad);
+            const int nwin = (g.nrings + 63) / 64;
+            const double vx = sin(g.theta) * cos(g.phi), vy = sin(g.theta) * sin(g.phi);
+            // the particle's windows are walked from a window that depends
+            // on the particle, so blocks rendering overlapping sky-covering
+            // footprints do not sweep the same rings' pixels with atomics
+            // at the same time
+            const int rot = (int)((it * 2654435761u) >> 8) % nwin;
+            double scale64 = 0.0;
+            for (int pass = 0; pass < 2; ++pass) {
+                double nacc = 0.0;
+                int T = 0;
+                for (int k = wid; k < nwin; k += 4) {
+                    const int win = k + rot < nwin ? k + rot : k + rot - nwin;
+                    const int Tw = setup_window<ACC == kAccF64>(g, nside, win, L, lane, 0);
+                    T += Tw;
+                    for (int j = 0; j < 64; ++j) {
+                        const int pre_start = j ? L.pre_end[j - 1] : 0;
+                        const int cnt = L.pre_end[j] - pre_start;
+                        if (cnt <= 0) continue;
+                        const float4 f = L.f[j];
+                        const float4 gq = L.gq[j];
+                        const int kref = L.kref[j];
+                        const int4 a = L.a[j];
+                        const double4 dd = L.d[j];
+                        float part = 0.0f;
+                        for (int t = lane; t < cnt; t += 64) {
+                            const int k = pre_start + t;
+                            if constexpr (ACC == kAccF64) {
+                                const double w = weight64(dd.x, dd.y, ((double)(k - a.w) + dd.z) * dd.w, g, vx, vy);
+                                if (pass == 0) nacc += w;
+                                else acc_add<ACC>(map, pix_addr(a, k), 0.0f, (w * scale64));
+                            } else {
+                                const float kf = (float)(k - kref);
+                                float w;
+                                if (g.mode == 0) w = weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, 0.0f, 0.0f, g.K4);
+                                else w = weight32<1>(fmaf(kf, f.w, f.z), f.x, gq.x, gq.y, gq.z, g.K4);
+                                if (pass == 0) part += w;
+                                else acc_add<ACC>(map, pix_addr(a, k), w * (float)scale64, 0.0);
+                            }
+                        }
+                        if constexpr (ACC != kAccF64) nacc += (double)part;
+                    }
+                    wave_sync();
+                }
+                if (pass == 0) {
+                    const double ws = wave_sum(nacc);
+                    if (lane == 0) {
+                        s_red[wid] = ws;
+                        s_cnt[wid] = T;
+                    }
+                    __syncthreads();
+                    const double norm = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+                    const int Ttot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+                    __syncthreads();
+                    if (Ttot < 2) {
+                        if (threadIdx.x == 0) {
+                            acc_point<ACC>(map, g.p0, g.flux);
+                            atomicAdd(&stats[kStatFallback], 1ull);
+                        }
+                        break;
+                    }
+                    if (ACC == kAccF64) scale64 = g.flux / norm;
+                    else scale64 = g.flux / norm;
+                    if (threadIdx.x == 0) {
+                        atomicAdd(&stats[kStatExt], 1ull);
+                        atomicAdd(&stats[kStatLarge], 1ull);
+                        atomicAdd(&stats[kStatContrib], (unsigned long long)Ttot);
+                        atomicAdd(&stats[kStatLargeContrib], (unsigned long long)Ttot);
+                    }
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -629,7 +673,7 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
                                   unsigned *counts, MapRef map, unsigned long long *stats,
                                   const LaunchCfg &cfg, hipStream_t st, hipEvent_t *ev, const GroupCfg *gc)
 {
-    k_fill_u32<<<1, 64, 0, st>>>(counts, 2);
+    k_fill_u32<<<1, 64, 0, st>>>(counts, 3);
     const int64_t want = (n + 255) / 256;
     const int pre_blocks = (int)(want < cfg.prepass_blocks ? want : cfg.prepass_blocks);
     if (ev) (void)hipEventRecord(ev[0], st);
