@@ -231,7 +231,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
         return bail("hipMalloc stats", e);
     c->gc.sort_blocks = dev_cus * 16;
     c->gc.group_blocks = dev_cus * DMANN_GROUP_BPC;
-    if ((e = alloc_group(c, 16, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
+    if ((e = alloc_group(c, 32, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);
     *out = c;

@@ -286,7 +286,7 @@ class Projector:
         _check(self._L.dmann_set_tuning(self._h, int(medium_blocks_per_cu), int(large_blocks_per_cu),
                                         int(large_pix)), self._h)
 
-    def set_grouping(self, k: int = 16, nside_c: int | None = None):
+    def set_grouping(self, k: int = 32, nside_c: int | None = None):
         """K neighbours per wavefront (0: one particle per wavefront)."""
         ns = self.params.nside
         _check(self._L.dmann_set_grouping(self._h, int(k), int(nside_c or max(1, ns // 8))), self._h)

@@ -167,7 +167,7 @@ int dmann_set_tuning(dmann_ctx *ctx, int medium_blocks_per_cu, int large_blocks_
  * particles are sorted by RING cell at nside_c (1..min(nside, 1024)), then
  * by footprint-size class, and K (1..64) neighbours share one wavefront and
  * an LDS window, cutting global atomics ~K-fold; k = 0 renders one particle
- * per wavefront with per-pixel atomics.  Defaults K = 16, nside_c =
+ * per wavefront with per-pixel atomics.  Defaults K = 32, nside_c =
  * nside / 8.  Ignored by the FP64 form. */
 int dmann_set_grouping(dmann_ctx *ctx, int k, int nside_c);
 
