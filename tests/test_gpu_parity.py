@@ -138,6 +138,30 @@ def _sky_spread(pw, n, seed, rad=0.06):
     return a
 
 
+@pytest.mark.parametrize("precision", ["fp64", "compensated", "fp32"])
+def test_sky_covering_size_classes(pw, orc, gpu, precision):
+    """900 particles at 5 kpc with footprint radii 2r log-uniform in
+    0.05 .. 3.5 rad (from ~100 pixels to the whole sky) at nside 128: the
+    sky-covering class (about half of them, more than one 256-particle
+    chunk) spans all of its kernel's size classes, which are rendered
+    largest first; every particle must be rendered exactly once."""
+    import numpy as np
+    print(f"case sky-covering size classes precision={precision}", flush=True)
+    rng = np.random.default_rng(21)
+    n = 900
+    rad = np.exp(rng.uniform(np.log(0.05), np.log(3.5), n))
+    recs = _sky_spread(pw, n, seed=22)
+    recs["hsmooth"] = (0.5 * rad * 5.0).astype(np.float32)
+    ref, ost = _run_oracle(orc, pw, recs, 128)
+    m, st = _run_gpu(pw, recs, 128, precision)
+    _check_counts(st, ost)
+    assert st["large"] > 256
+    rep = rel_err_report(m, ref)
+    print(rep, st["large"])
+    tol, ttol = TOL[precision]
+    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+
+
 @pytest.mark.parametrize("precision", ["compensated", "fp32"])
 def test_grouping_wide_windows(pw, orc, gpu, precision):
     """One group of 60 particles spread over the whole sky (sort-cell nside
