@@ -26,6 +26,8 @@ namespace dmann {
 
 // size classes of the sky-covering kernel's largest-first order
 constexpr int kLargeClasses = 10;
+// particles per thread per pre-pass iteration
+constexpr int kPrepassItems = 2;
 
 // ---------------------------------------------------------------- pre-pass
 
@@ -37,29 +39,69 @@ __global__ __launch_bounds__(256) void k_prepass(const Particle *__restrict__ pa
                                                  unsigned *__restrict__ counts, MapRef map,
                                                  unsigned long long *__restrict__ stats)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // Each block takes kPrepassItems x 256 particles per iteration (item j
+    // of thread t is particle base + j 256 + t, so loads stay coalesced)
+    // and appends its medium and sky-covering records with one global
+    // atomic each: the two append counters are single addresses, and one
+    // atomic per wave and 64 particles made them the bottleneck.
+    constexpr int J = kPrepassItems;
+    __shared__ unsigned s_cnt[2][4], s_base[2];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * J;
     unsigned long long n_kept = 0, n_sub = 0, n_bad = 0;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
-        const int64_t i = base + threadIdx.x;
-        int cls = kClsSkip;
-        DiscRec rec;
-        if (i < n) {
-            const Particle q = parts[i];
-            bool bad;
-            const int c = prepass_one(P, q, rec, bad);
-            if (bad) n_bad++;
-            if (c != kClsSkip) n_kept++;
-            if (c == kClsPoint) {
-                n_sub++;
-                acc_point<ACC>(map, rec.p0, rec.flux);
-            } else if (c == kClsMedium || c == kClsLarge) {
-                cls = c;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x * J; base < n; base += stride) {
+        DiscRec rec[J];
+        int cls[J];
+        uint64_t mm[J], ml[J];
+        unsigned tm = 0, tl = 0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int64_t i = base + (int64_t)j * blockDim.x + threadIdx.x;
+            cls[j] = kClsSkip;
+            if (i < n) {
+                const Particle q = parts[i];
+                bool bad;
+                const int c = prepass_one(P, q, rec[j], bad);
+                if (bad) n_bad++;
+                if (c != kClsSkip) n_kept++;
+                if (c == kClsPoint) {
+                    n_sub++;
+                    acc_point<ACC>(map, rec[j].p0, rec[j].flux);
+                } else if (c == kClsMedium || c == kClsLarge) {
+                    cls[j] = c;
+                }
             }
+            mm[j] = __ballot(cls[j] == kClsMedium);
+            ml[j] = __ballot(cls[j] == kClsLarge);
+            tm += (unsigned)__popcll(mm[j]);
+            tl += (unsigned)__popcll(ml[j]);
         }
-        const unsigned im = wave_append(&counts[0], cls == kClsMedium);
-        const unsigned il = wave_append(&counts[1], cls == kClsLarge);
-        if (cls == kClsMedium) recs[im] = rec;
-        if (cls == kClsLarge) recs[n - 1 - (int64_t)il] = rec;
+        if (lane == 0) {
+            s_cnt[0][wid] = tm;
+            s_cnt[1][wid] = tl;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            const unsigned t = s_cnt[threadIdx.x][0] + s_cnt[threadIdx.x][1] + s_cnt[threadIdx.x][2] +
+                               s_cnt[threadIdx.x][3];
+            s_base[threadIdx.x] = t ? atomicAdd(&counts[threadIdx.x], t) : 0u;
+        }
+        __syncthreads();
+        unsigned om = s_base[0], ol = s_base[1];
+        for (int w = 0; w < wid; ++w) {
+            om += s_cnt[0][w];
+            ol += s_cnt[1][w];
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (cls[j] == kClsMedium) recs[om + (unsigned)__popcll(mm[j] & below)] = rec[j];
+            if (cls[j] == kClsLarge) recs[n - 1 - (int64_t)(ol + (unsigned)__popcll(ml[j] & below))] = rec[j];
+            om += (unsigned)__popcll(mm[j]);
+            ol += (unsigned)__popcll(ml[j]);
+        }
+        __syncthreads(); // s_cnt / s_base are rewritten by the next iteration
     }
     wave_count(&stats[kStatKept], n_kept);
     wave_count(&stats[kStatSub], n_sub);
@@ -674,7 +716,7 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
                                   const LaunchCfg &cfg, hipStream_t st, hipEvent_t *ev, const GroupCfg *gc)
 {
     k_fill_u32<<<1, 64, 0, st>>>(counts, 3);
-    const int64_t want = (n + 255) / 256;
+    const int64_t want = (n + 256 * kPrepassItems - 1) / (256 * kPrepassItems);
     const int pre_blocks = (int)(want < cfg.prepass_blocks ? want : cfg.prepass_blocks);
     if (ev) (void)hipEventRecord(ev[0], st);
     if (pre_blocks > 0)
