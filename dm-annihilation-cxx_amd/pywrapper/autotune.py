@@ -2,8 +2,9 @@
 
 Times candidate settings on a particle sample with HIP events (through
 torch's stream) and keeps the fastest.  Single map: persistent workgroups
-per CU of the two footprint kernels and the footprint size that sends a
-particle to the workgroup kernel.  Batch: particles per work item and
+per CU of the two footprint kernels, the footprint size that sends a
+particle to the workgroup kernel and, optionally, the group size of the
+grouped medium-class kernel.  Batch: particles per work item and
 workgroups per CU of the persistent kernel.
 """
 from __future__ import annotations
@@ -27,21 +28,29 @@ def _time(fn, reps: int = 2) -> float:
 
 
 def tune_projector(projector, parts, medium=(4, 8, 16), large=(2, 4, 8), large_pix=(16384, 65536, 262144),
-                   reps: int = 2):
-    """Returns (best settings dict, {settings: ms}) and applies the best."""
+                   group=(None,), reps: int = 2):
+    """Returns (best settings dict, {settings: ms}) and applies the best.
+
+    group: candidate group sizes K of the grouped medium-class kernel
+    (Projector.set_grouping; None keeps the current one)."""
     results = {}
 
     def run():
         projector.reset()
         projector.render(parts)
 
-    for m, l, lp in itertools.product(medium, large, large_pix):
+    for k, m, l, lp in itertools.product(group, medium, large, large_pix):
+        if k is not None:
+            projector.set_grouping(k)
         projector.set_tuning(m, l, lp)
-        results[(m, l, lp)] = _time(run, reps)
+        results[(m, l, lp, k)] = _time(run, reps)
     best = min(results, key=results.get)
-    projector.set_tuning(*best)
+    if best[3] is not None:
+        projector.set_grouping(best[3])
+    projector.set_tuning(*best[:3])
     projector.reset()
-    return {"medium_blocks_per_cu": best[0], "large_blocks_per_cu": best[1], "large_pix": best[2]}, results
+    return {"medium_blocks_per_cu": best[0], "large_blocks_per_cu": best[1], "large_pix": best[2],
+            "group": best[3]}, results
 
 
 def tune_batch(batch, parts, chunks=(16, 64, 256), blocks=(4, 8, 16), reps: int = 2):

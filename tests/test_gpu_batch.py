@@ -57,8 +57,9 @@ def test_autotuner_runs(pw, gpu):
     parts = pw.synth.make("nfw_sub100_1e7", n=20000).to(gpu)
     p = pw.params_for_observer(128, "compensated")
     with pw.Projector(p, parts.shape[0]) as pr:
-        best, res = pw.autotune.tune_projector(pr, parts, medium=(4, 8), large=(4,), large_pix=(4096, 65536))
-        assert len(res) == 4 and best["medium_blocks_per_cu"] in (4, 8)
+        best, res = pw.autotune.tune_projector(pr, parts, medium=(4, 8), large=(4,), large_pix=(4096, 65536),
+                                               group=(16, 32))
+        assert len(res) == 8 and best["medium_blocks_per_cu"] in (4, 8) and best["group"] in (16, 32)
         pr.reset()
         pr.render(parts)
         m1 = pr.get_map()
