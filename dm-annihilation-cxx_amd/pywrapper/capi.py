@@ -381,6 +381,17 @@ def host_footprint(params: DmannParams, rec):
         cap = int(n)
 
 
+def host_ring_check(params: DmannParams, rec) -> int:
+    """For a medium-class particle: the number of its rings whose ranges in
+    the grouped kernel's cheap form (ring_lite) differ from the query_disc
+    construction (ring_setup); -2 for other classes."""
+    L = load_library()
+    r = np.ascontiguousarray(np.asarray(rec).reshape(-1)[:1]).view(np.float32)
+    if r.size != 12:
+        r = np.ascontiguousarray(np.asarray(rec, dtype=np.float32).reshape(12))
+    return int(L.dmann_host_ring_check(C.byref(params), C.c_void_p(r.ctypes.data)))
+
+
 # ------------------------------------------------------------------ batch
 
 class Batch:

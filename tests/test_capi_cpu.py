@@ -89,6 +89,27 @@ def test_host_footprint_matches_oracle(pw, orc, tmp_path, workload, nside, m):
     assert worst <= 5e-6
 
 
+@pytest.mark.parametrize("nside", [64, 512, 2048, 4096])
+def test_ring_lite_matches_ring_setup(pw, tmp_path, nside):
+    """The grouped kernel's ring ranges -- an FP32 half-width with an FP64
+    recomputation when a column boundary lies within its error bound --
+    equal the query_disc construction's ring by ring: column counts, first
+    columns and offsets exactly, FP32 terms to a few units of rounding (a
+    3e-5 relative error planted in the half-width fails this at nside
+    512)."""
+    ini = pw.write_ini(str(tmp_path / "o.ini"), nside=nside)
+    p, _, _ = pw.params_from_ini(ini, "fp32")
+    recs = pw.synth.to_records(pw.synth.make("nfw_sub2000_near_1p1e9", n=1500))
+    checked = 0
+    for q in recs:
+        m = pw.capi.host_ring_check(p, q)
+        if m == -2:
+            continue
+        assert m == 0
+        checked += 1
+    assert checked > 1000
+
+
 def test_host_footprint_edge_geometries(pw, orc, tmp_path):
     """Footprints over a pole, across the phi = 0 seam, covering the whole
     sky, and tiny ones, placed by inverting the observer rotation."""
