@@ -138,6 +138,29 @@ def _sky_spread(pw, n, seed, rad=0.06):
     return a
 
 
+_HEADLINE = {}  # nside -> (records, oracle map, oracle statistics)
+
+
+@pytest.mark.parametrize("nside", [2048, 4096])
+@pytest.mark.parametrize("precision", ["fp64", "compensated"])
+def test_headline_model_sample_high_nside(pw, orc, gpu, precision, nside):
+    """3000 particles of the 1.1e9-particle model (one of them near the
+    observer, covering much of the sky) at the benchmark's nside 2048 and
+    the precision-at-scale nside 4096, against the oracle."""
+    print(f"case headline sample nside={nside} precision={precision}", flush=True)
+    if nside not in _HEADLINE:
+        hm = pw.synth.model("nfw_sub2000_near_1p1e9", "cpu")
+        recs = pw.synth.to_records(hm.sample(3000))
+        _HEADLINE[nside] = (recs,) + tuple(_run_oracle(orc, pw, recs, nside))
+    recs, ref, ost = _HEADLINE[nside]
+    m, st = _run_gpu(pw, recs, nside, precision)
+    _check_counts(st, ost)
+    rep = rel_err_report(m, ref)
+    print(rep, st["large"])
+    tol, ttol = TOL[precision]
+    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+
+
 @pytest.mark.parametrize("precision", ["fp64", "compensated", "fp32"])
 def test_sky_covering_size_classes(pw, orc, gpu, precision):
     """900 particles at 5 kpc with footprint radii 2r log-uniform in
