@@ -19,9 +19,6 @@
 
 // resident single-wave blocks per CU of the grouped kernel (extra blocks
 // find the group counter exhausted and exit)
-#ifndef DMANN_SIDE_STREAM
-#define DMANN_SIDE_STREAM 1
-#endif
 #ifndef DMANN_GROUP_BPC
 #define DMANN_GROUP_BPC 8
 #endif
@@ -46,7 +43,6 @@ struct dmann_ctx {
     int cus = 0;
     bool timing = false;
     std::vector<hipEvent_t> events; // 5 per timed render call
-    SideStream side{nullptr, nullptr, nullptr}; // the sky-covering kernel's stream
     // halo cores of the ANN_CORE_NOSOM plug-in (device copies)
     float4 *d_halos = nullptr;
     int *d_cell_start = nullptr;
@@ -134,10 +130,6 @@ static void free_all(dmann_ctx *c)
 {
     free_group(c);
     free_halos(c);
-    if (c->side.stream) hipStreamDestroy(c->side.stream);
-    if (c->side.fork) hipEventDestroy(c->side.fork);
-    if (c->side.join) hipEventDestroy(c->side.join);
-    c->side = SideStream{nullptr, nullptr, nullptr};
     hipFree(c->d_parts);
     hipFree(c->d_recs);
     hipFree(c->d_counts);
@@ -240,13 +232,6 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     c->gc.sort_blocks = dev_cus * 16;
     c->gc.group_blocks = dev_cus * DMANN_GROUP_BPC;
     if ((e = alloc_group(c, 32, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
-#if DMANN_SIDE_STREAM
-    if ((e = hipStreamCreateWithFlags(&c->side.stream, hipStreamNonBlocking)) != hipSuccess)
-        return bail("hipStreamCreate", e);
-    if ((e = hipEventCreateWithFlags(&c->side.fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&c->side.join, hipEventDisableTiming)) != hipSuccess)
-        return bail("hipEventCreate", e);
-#endif
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);
     *out = c;
@@ -387,7 +372,7 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
         ev = &c->events[c->events.size() - 5];
     }
     HIPCHK(c, launch_render(c->acc, reinterpret_cast<const Particle *>(dev_parts), n, c->P, c->d_recs,
-                            c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev, &c->gc, &c->side));
+                            c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev, &c->gc));
     return DMANN_OK;
 }
 
@@ -417,10 +402,7 @@ int dmann_timing(dmann_ctx *c, int enable, double *ms, int *calls)
         for (size_t i = 0; i + 4 < c->events.size(); i += 5)
             for (int k = 0; k < 4; k++) {
                 float t = 0;
-                // the sort follows the pre-pass directly when the
-                // sky-covering kernel runs on the side stream
-                const int k0 = k == 2 && c->side.stream ? 1 : k;
-                HIPCHK(c, hipEventElapsedTime(&t, c->events[i + k0], c->events[i + k + 1]));
+                HIPCHK(c, hipEventElapsedTime(&t, c->events[i + k], c->events[i + k + 1]));
                 acc[k] += t;
             }
         if (ms)
@@ -447,7 +429,7 @@ int dmann_render(dmann_ctx *c, const dmann_particle *host_parts, int64_t n)
     if (!c->d_parts) HIPCHK(c, hipMalloc(&c->d_parts, c->max_parts * sizeof(Particle)));
     HIPCHK(c, hipMemcpyAsync(c->d_parts, host_parts, n * sizeof(Particle), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_render(c->acc, c->d_parts, n, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg,
-                            c->stream, nullptr, &c->gc, &c->side));
+                            c->stream, nullptr, &c->gc));
     // the staging buffer is reused by the next call: wait for this one
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return DMANN_OK;

@@ -112,22 +112,11 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
                         unsigned long long *stats, hipStream_t st,
                         hipEvent_t sorted = nullptr);
 
-// A stream of the context's own (default priority) on which the
-// sky-covering kernel runs beside the medium-class kernels, with the two
-// events that fork it from and join it back to the caller's stream.
-struct SideStream {
-    hipStream_t stream;
-    hipEvent_t fork, join;
-};
-
-// ev (optional): 5 events recorded before the pre-pass, after it, after the
-// sky-covering kernel (on its stream), after the sort and after the medium
-// kernel.  Intervals: pre-pass 0-1, sky-covering 1-2, sort 1-3 with a side
-// stream (2-3 without), medium 3-4.
+// ev (optional): 4 events recorded before the pre-pass, after it, after the
+// sky-covering kernel and after the medium kernel.
 hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                          unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
-                         hipStream_t st, hipEvent_t *ev = nullptr, const GroupCfg *gc = nullptr,
-                         const SideStream *side = nullptr);
+                         hipStream_t st, hipEvent_t *ev = nullptr, const GroupCfg *gc = nullptr);
 hipError_t launch_adjoint(const Particle *d_parts, int64_t n, const DevParams &P, const double *gmap, double *out,
                           int blocks, hipStream_t st);
 hipError_t launch_batch(int acc, const Particle *d_parts, int64_t n, const DevParams *Ps, const MapRef *maps,

@@ -713,8 +713,7 @@ __global__ void k_finalize(MapRef map, int64_t npix, double dOmega, double *__re
 template <int ACC>
 static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                                   unsigned *counts, MapRef map, unsigned long long *stats,
-                                  const LaunchCfg &cfg, hipStream_t st, hipEvent_t *ev, const GroupCfg *gc,
-                                  const SideStream *side)
+                                  const LaunchCfg &cfg, hipStream_t st, hipEvent_t *ev, const GroupCfg *gc)
 {
     k_fill_u32<<<1, 64, 0, st>>>(counts, 3);
     const int64_t want = (n + 256 * kPrepassItems - 1) / (256 * kPrepassItems);
@@ -723,19 +722,8 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
     if (pre_blocks > 0)
         k_prepass<ACC><<<pre_blocks, 256, 0, st>>>(d_parts, n, P, recs, counts, map, stats);
     if (ev) (void)hipEventRecord(ev[1], st);
-    // The sky-covering kernel is bound by its atomics and the medium-class
-    // kernels by VALU work: with a side stream they run concurrently (both
-    // only add into the map, and read disjoint record ranges), joined
-    // before the call returns to the caller's stream.
-    hipStream_t lst = st;
-    if (side && side->stream) {
-        hipError_t e = hipEventRecord(side->fork, st);
-        if (e == hipSuccess) e = hipStreamWaitEvent(side->stream, side->fork, 0);
-        if (e != hipSuccess) return e;
-        lst = side->stream;
-    }
-    k_disc_large<ACC><<<cfg.large_blocks, 256, 0, lst>>>(recs, n, counts, P.nside, P.footprint_pad, map, stats);
-    if (ev) (void)hipEventRecord(ev[2], lst);
+    k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, P.footprint_pad, map, stats);
+    if (ev) (void)hipEventRecord(ev[2], st);
     if (ACC != kAccF64 && gc && gc->k > 0) {
         const hipError_t e =
             launch_group(ACC, recs, counts, P.nside, P.footprint_pad, *gc, map, stats, st, ev ? ev[3] : nullptr);
@@ -745,22 +733,17 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
         k_disc_medium<ACC><<<cfg.medium_blocks, 256, 0, st>>>(recs, counts, P.nside, P.footprint_pad, map, stats);
     }
     if (ev) (void)hipEventRecord(ev[4], st);
-    if (lst != st) {
-        hipError_t e = hipEventRecord(side->join, lst);
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, side->join, 0);
-        if (e != hipSuccess) return e;
-    }
     return hipGetLastError();
 }
 
 hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                          unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
-                         hipStream_t st, hipEvent_t *ev, const GroupCfg *gc, const SideStream *side)
+                         hipStream_t st, hipEvent_t *ev, const GroupCfg *gc)
 {
     switch (acc) {
-    case kAccF32: return launch_render_t<kAccF32>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc, side);
-    case kAccComp: return launch_render_t<kAccComp>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc, side);
-    default: return launch_render_t<kAccF64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc, side);
+    case kAccF32: return launch_render_t<kAccF32>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
+    case kAccComp: return launch_render_t<kAccComp>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
+    default: return launch_render_t<kAccF64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
     }
 }
 

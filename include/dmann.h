@@ -190,16 +190,13 @@ int dmann_synchronize(dmann_ctx *ctx);
 int dmann_adjoint_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t n, const double *dev_grad_map,
                          double *dev_grad_parts);
 
-/* Per-kernel timing with HIP events.  Reads the summed durations (ms) of
- * the kernels of the render calls made since the last call -- ms[0]
- * pre-pass, ms[1] sky-covering kernel, ms[2] sort of the medium records by
- * coarse cell, ms[3] medium footprint kernel -- and their number (both may
- * be NULL), then turns event recording on (enable != 0) or off for the
- * following render calls.  The sky-covering kernel runs on a stream of the
- * context's own, forked from and joined back to the context's stream, so
- * the sort may start before it ends: ms[1] and ms[2] both count from the
- * end of the pre-pass.  A measurement aid with no counterpart in the
- * reference. */
+/* Per-kernel timing with HIP events on the context's stream.  Reads the
+ * summed durations (ms) of the kernels of the render calls made since the
+ * last call -- ms[0] pre-pass, ms[1] sky-covering kernel, ms[2] sort of the
+ * medium records by coarse cell, ms[3] medium footprint kernel -- and their
+ * number (both may be NULL), then turns event recording on (enable != 0) or
+ * off for the following render calls.  A measurement aid with no
+ * counterpart in the reference. */
 int dmann_timing(dmann_ctx *ctx, int enable, double *ms, int *calls);
 
 /* Writes the finished map, RING order, npix doubles, already divided by
