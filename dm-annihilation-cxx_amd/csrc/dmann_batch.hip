@@ -1,8 +1,11 @@
 // Batched maps: many observers (each with its own nside) rendered from one
-// particle set in one persistent launch, the whole render -- map clears,
-// the persistent kernel, the FP64 finalisation of every map -- captured once
-// in a hipGraph and replayed.  No counterpart in the reference, which renders
-// one observer per run.
+// particle set, the whole render -- map clears, the footprint work, the FP64
+// finalisation of every map -- captured once in a hipGraph and replayed.
+// Two schedules: by default each map runs the single-map pipeline (pre-pass,
+// sky-covering kernel, sort, grouped kernel) on buffers shared by the maps;
+// dmann_batch_set_tuning with chunk >= 1 selects one persistent launch over
+// (map, particle chunk) work items with per-pixel atomics.  No counterpart in
+// the reference, which renders one observer per run.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -17,6 +20,10 @@
 
 using namespace dmann;
 
+#ifndef DMANN_GROUP_BPC
+#define DMANN_GROUP_BPC 8
+#endif
+
 struct dmann_batch {
     int device = 0;
     int acc = kAccF32;
@@ -30,9 +37,18 @@ struct dmann_batch {
     MapRef *d_maps = nullptr;
     unsigned long long *d_work = nullptr;
     unsigned long long *d_stats = nullptr;
-    int chunk = 64;
+    int chunk = 0; // 0: per-map pipeline; >= 1: persistent launch, particles per work item
     int blocks = 0;
     int cus = 256;
+    // per-map pipeline: each map's parameters (single-map thresholds) and
+    // the buffers the maps share
+    std::vector<DevParams> Pmap;
+    DiscRec *recs = nullptr;
+    int64_t recs_cap = 0;
+    unsigned *counts = nullptr;
+    LaunchCfg cfg{};
+    GroupCfg gc{};
+    int nside_c_max = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void *graph_parts = nullptr;
@@ -82,9 +98,60 @@ static void batch_free(dmann_batch *b)
     (void)hipFree(b->d_maps);
     (void)hipFree(b->d_work);
     (void)hipFree(b->d_stats);
+    (void)hipFree(b->recs);
+    (void)hipFree(b->counts);
+    (void)hipFree(b->gc.keys);
+    (void)hipFree(b->gc.idx);
+    (void)hipFree(b->gc.hist);
+    (void)hipFree(b->gc.sums);
+    (void)hipFree(b->gc.work);
+    (void)hipFree(b->gc.tab_f);
+    (void)hipFree(b->gc.tab_i);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     b->maps.clear();
     b->out.clear();
+}
+
+// Sort-cell resolution of a map in the per-map pipeline (the single-map
+// default: nside / 8).
+static int sort_nside(int nside)
+{
+    const int nc = nside >= 8 ? nside / 8 : 1;
+    return nc < kMaxSortNside ? nc : kMaxSortNside;
+}
+
+// Per-map pipeline buffers for n particles (records, sort keys and index
+// grow with n; the sort histogram is sized for the finest map), allocated
+// before any capture.
+static hipError_t ensure_pipeline(dmann_batch *b, int64_t n)
+{
+    hipError_t e;
+    if (!b->counts && (e = hipMalloc(&b->counts, 4 * sizeof(unsigned))) != hipSuccess) return e;
+    const int64_t cap = n > 1 ? n : 1;
+    if (b->recs_cap < cap) {
+        (void)hipFree(b->recs);
+        (void)hipFree(b->gc.keys);
+        (void)hipFree(b->gc.idx);
+        b->recs = nullptr;
+        b->gc.keys = b->gc.idx = nullptr;
+        b->recs_cap = 0;
+        if ((e = hipMalloc(&b->recs, cap * sizeof(DiscRec))) != hipSuccess) return e;
+        if (b->acc != kAccF64) {
+            if ((e = hipMalloc(&b->gc.keys, cap * sizeof(unsigned))) != hipSuccess) return e;
+            if ((e = hipMalloc(&b->gc.idx, cap * sizeof(unsigned))) != hipSuccess) return e;
+        }
+        b->recs_cap = cap;
+    }
+    if (b->acc != kAccF64 && !b->gc.hist) {
+        const int64_t nbins = 12LL * b->nside_c_max * b->nside_c_max * kSizeBuckets;
+        if ((e = hipMalloc(&b->gc.hist, nbins * sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipMalloc(&b->gc.sums, ((nbins + 1023) / 1024) * sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipMalloc(&b->gc.work, sizeof(unsigned))) != hipSuccess) return e;
+        const size_t rows = (size_t)b->gc.group_blocks * (size_t)b->gc.k * 64;
+        if ((e = hipMalloc(&b->gc.tab_f, rows * sizeof(float4))) != hipSuccess) return e;
+        if ((e = hipMalloc(&b->gc.tab_i, rows * sizeof(int4))) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Enqueues one full render of the batch on its stream.
@@ -95,7 +162,17 @@ static hipError_t enqueue(dmann_batch *b, const Particle *parts, int64_t n)
         if ((e = launch_clear(b->acc, b->maps[m], b->npix[m], b->d_stats + (size_t)m * kNumStats, b->stream)) !=
             hipSuccess)
             return e;
-    if (n > 0 &&
+    if (n > 0 && b->chunk == 0) {
+        // the single-map pipeline, map after map, on the shared buffers
+        for (int m = 0; m < b->nmaps; m++) {
+            GroupCfg gc = b->gc;
+            gc.nside_c = sort_nside(b->Pmap[m].nside);
+            if ((e = launch_render(b->acc, parts, n, b->Pmap[m], b->recs, b->counts, b->maps[m],
+                                   b->d_stats + (size_t)m * kNumStats, b->cfg, b->stream, nullptr,
+                                   b->acc == kAccF64 ? nullptr : &gc)) != hipSuccess)
+                return e;
+        }
+    } else if (n > 0 &&
         (e = launch_batch(b->acc, parts, n, b->d_P, b->d_maps, b->nmaps, b->chunk, b->d_work, b->d_stats, b->blocks,
                           b->stream)) != hipSuccess)
         return e;
@@ -144,12 +221,23 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         b->cus = prop.multiProcessorCount;
     b->blocks = b->cus * 8;
+    b->cfg.prepass_blocks = b->cus * 64;
+    b->cfg.medium_blocks = b->cus * 8;
+    b->cfg.large_blocks = b->cus * 4;
+    b->gc.k = 32;
+    b->gc.sort_blocks = b->cus * 16;
+    b->gc.group_blocks = b->cus * DMANN_GROUP_BPC;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
     for (int m = 0; m < nmaps; m++) {
         const dmann_params &p = params[m];
         DevParams P;
         fill_devparams(&p, P, 1 << 16);
         b->P.push_back(P);
+        DevParams Pm;
+        fill_devparams(&p, Pm, 1 << 22); // the single-map thresholds
+        b->Pmap.push_back(Pm);
+        const int nc = sort_nside(p.nside);
+        if (nc > b->nside_c_max) b->nside_c_max = nc;
         b->npix.push_back(P.npix);
         MapRef r{nullptr, nullptr, nullptr};
         double *o = nullptr;
@@ -181,8 +269,8 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
 int dmann_batch_set_tuning(dmann_batch *b, int chunk, int blocks_per_cu)
 {
     if (!b) return DMANN_ERR_ARG;
-    if (chunk < 1 || chunk > (1 << 20) || blocks_per_cu < 1 || blocks_per_cu > 64)
-        return bfail(b, DMANN_ERR_ARG, "tuning out of range (chunk 1..2^20, blocks per CU 1..64)");
+    if (chunk < 0 || chunk > (1 << 20) || blocks_per_cu < 1 || blocks_per_cu > 64)
+        return bfail(b, DMANN_ERR_ARG, "tuning out of range (chunk 0..2^20, blocks per CU 1..64)");
     b->chunk = chunk;
     b->blocks = b->cus * blocks_per_cu;
     drop_graph(b);
@@ -195,6 +283,8 @@ int dmann_batch_render_device(dmann_batch *b, const dmann_particle *dev_parts, i
     if (n < 0 || (n > 0 && !dev_parts)) return bfail(b, DMANN_ERR_ARG, "bad particle arguments");
     BCHK(b, hipSetDevice(b->device));
     const Particle *parts = reinterpret_cast<const Particle *>(dev_parts);
+    if (b->chunk == 0 && n > b->recs_cap) drop_graph(b); // its buffers are reallocated
+    if (b->chunk == 0) BCHK(b, ensure_pipeline(b, n));
     if (!use_graph) {
         BCHK(b, enqueue(b, parts, n));
         return DMANN_OK;
@@ -257,8 +347,9 @@ int dmann_batch_get_stats(dmann_batch *b, int i, dmann_stats *st)
     st->fallback = (int64_t)h[kStatFallback];
     st->extended = (int64_t)h[kStatExt];
     st->contributions = (int64_t)h[kStatContrib];
-    st->large = 0;
-    st->large_contributions = 0;
+    // (the persistent launch has no sky-covering class)
+    st->large = b->chunk == 0 ? (int64_t)h[kStatLarge] : 0;
+    st->large_contributions = b->chunk == 0 ? (int64_t)h[kStatLargeContrib] : 0;
     return DMANN_OK;
 }
 

@@ -4,8 +4,8 @@ Times candidate settings on a particle sample with HIP events (through
 torch's stream) and keeps the fastest.  Single map: persistent workgroups
 per CU of the two footprint kernels, the footprint size that sends a
 particle to the workgroup kernel and, optionally, the group size of the
-grouped medium-class kernel.  Batch: particles per work item and
-workgroups per CU of the persistent kernel.
+grouped medium-class kernel.  Batch: the per-map pipeline, or particles per
+work item and workgroups per CU of the persistent kernel.
 """
 from __future__ import annotations
 
@@ -53,9 +53,13 @@ def tune_projector(projector, parts, medium=(4, 8, 16), large=(2, 4, 8), large_p
             "group": best[3]}, results
 
 
-def tune_batch(batch, parts, chunks=(16, 64, 256), blocks=(4, 8, 16), reps: int = 2):
+def tune_batch(batch, parts, chunks=(0, 16, 64, 256), blocks=(4, 8, 16), reps: int = 2):
+    """chunk 0 is the per-map pipeline (blocks does not apply to it);
+    chunk >= 1 the persistent launch with that many particles per item."""
     results = {}
     for c, b in itertools.product(chunks, blocks):
+        if c == 0 and b != blocks[0]:
+            continue
         batch.set_tuning(c, b)
         results[(c, b)] = _time(lambda: batch.render(parts, use_graph=True), reps)
     best = min(results, key=results.get)

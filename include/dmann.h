@@ -224,12 +224,18 @@ void dmann_destroy(dmann_ctx *ctx);
  * one particle set) ----
  * All maps of a batch share the particles and the precision; each has its
  * own observer, rotation and nside (params[i]).  A render computes every map
- * afresh (clears, one persistent launch over (map, particle-chunk) work
- * items, FP64 finalisation), and with use_graph != 0 it is captured once in
- * a hipGraph per (particle pointer, n) and replayed. */
+ * afresh -- clears, the footprint work, FP64 finalisation -- and with
+ * use_graph != 0 it is captured once in a hipGraph per (particle pointer, n)
+ * and replayed.  The footprint work is, by default, the single-map pipeline
+ * (pre-pass, sky-covering kernel, sort, grouped kernel) map after map on
+ * buffers the maps share (records for n particles, allocated by the first
+ * render with that n); or one persistent launch over (map, particle-chunk)
+ * work items with per-pixel atomics (dmann_batch_set_tuning, chunk >= 1). */
 typedef struct dmann_batch dmann_batch;
 int dmann_batch_create(dmann_batch **b, const dmann_params *params, int nmaps, int device);
-/* particles per work item (1..2^20) and persistent workgroups per CU (1..64) */
+/* chunk 0: the per-map pipeline (default); chunk 1..2^20: the persistent
+ * launch with that many particles per work item and blocks_per_cu (1..64)
+ * persistent workgroups per CU */
 int dmann_batch_set_tuning(dmann_batch *b, int chunk, int blocks_per_cu);
 int dmann_batch_render_device(dmann_batch *b, const dmann_particle *dev_parts, int64_t n, int use_graph);
 int dmann_batch_synchronize(dmann_batch *b);

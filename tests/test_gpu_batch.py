@@ -1,5 +1,6 @@
 """Batched maps: observers on the 8 kpc circle in the z = 0 plane, mixed
-nside, one persistent launch replayed from a hipGraph; every map is checked
+nside, replayed from a hipGraph -- the per-map pipeline and the persistent
+launch; every map is checked
 against the single-observer projector on the same particles (FP64: 1e-12
 per pixel; compensated: 2e-6 per pixel, different atomic orders), graph
 replays against the first replay, and the autotuner runs.
@@ -19,16 +20,20 @@ def _observer_params(pw, k, nmaps, nside, precision):
     return pw.params_for_observer(nside, precision, obs_pos=(8 * math.cos(a), 8 * math.sin(a), 0.0))
 
 
+@pytest.mark.parametrize("chunk", [0, 64])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("compensated", 2e-6)])
-def test_batch_matches_single_observer(pw, gpu, precision, tol):
+def test_batch_matches_single_observer(pw, gpu, precision, tol, chunk):
+    """chunk 0: the per-map pipeline (default); 64: the persistent launch."""
     import torch
-    print(f"case batch precision={precision}", flush=True)
+    print(f"case batch precision={precision} chunk={chunk}", flush=True)
     parts = pw.synth.make("nfw_sub100_1e7", n=20000).to(gpu)
     rng = np.random.default_rng(2)
     nmaps = 8
     nsides = [int(x) for x in rng.choice([64, 128, 256], nmaps)]
     plist = [_observer_params(pw, k, nmaps, nsides[k], precision) for k in range(nmaps)]
     with pw.Batch(plist) as b:
+        if chunk:
+            b.set_tuning(chunk, 8)
         b.render(parts, use_graph=True)
         b.synchronize()
         maps = [b.get_map(i) for i in range(nmaps)]
@@ -69,6 +74,6 @@ def test_autotuner_runs(pw, gpu):
     assert rel_err_report(m1, m0)["max_rel"] <= 2e-6
     plist = [_observer_params(pw, k, 4, 64, "fp32") for k in range(4)]
     with pw.Batch(plist) as b:
-        best, res = pw.autotune.tune_batch(b, parts, chunks=(16, 64), blocks=(8,))
-        assert best["chunk"] in (16, 64)
+        best, res = pw.autotune.tune_batch(b, parts, chunks=(0, 16, 64), blocks=(8,))
+        assert len(res) == 3 and best["chunk"] in (0, 16, 64)
     torch.cuda.synchronize()
