@@ -28,7 +28,9 @@ struct dmann_batch {
     int device = 0;
     int acc = kAccF32;
     int nmaps = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the batch's own stream (graph capture and replay)
+    hipStream_t caller = nullptr;   // the caller's stream: renders wait for it and it waits for them
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::vector<DevParams> P;
     std::vector<MapRef> maps;
     std::vector<double *> out;
@@ -107,6 +109,9 @@ static void batch_free(dmann_batch *b)
     (void)hipFree(b->gc.work);
     (void)hipFree(b->gc.tab_f);
     (void)hipFree(b->gc.tab_i);
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    if (b->join_ev) (void)hipEventDestroy(b->join_ev);
+    b->fork_ev = b->join_ev = nullptr;
     if (b->stream) (void)hipStreamDestroy(b->stream);
     b->maps.clear();
     b->out.clear();
@@ -228,6 +233,8 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
     b->gc.sort_blocks = b->cus * 16;
     b->gc.group_blocks = b->cus * DMANN_GROUP_BPC;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
+    if ((e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming)) != hipSuccess) return bail("event", e);
+    if ((e = hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming)) != hipSuccess) return bail("event", e);
     for (int m = 0; m < nmaps; m++) {
         const dmann_params &p = params[m];
         DevParams P;
@@ -285,8 +292,15 @@ int dmann_batch_render_device(dmann_batch *b, const dmann_particle *dev_parts, i
     const Particle *parts = reinterpret_cast<const Particle *>(dev_parts);
     if (b->chunk == 0 && n > b->recs_cap) drop_graph(b); // its buffers are reallocated
     if (b->chunk == 0) BCHK(b, ensure_pipeline(b, n));
+    // the render reads particles the caller's stream may still be writing,
+    // and the caller's later work reads the maps: fork from and join back
+    // into the caller's stream
+    BCHK(b, hipEventRecord(b->fork_ev, b->caller));
+    BCHK(b, hipStreamWaitEvent(b->stream, b->fork_ev, 0));
     if (!use_graph) {
         BCHK(b, enqueue(b, parts, n));
+        BCHK(b, hipEventRecord(b->join_ev, b->stream));
+        BCHK(b, hipStreamWaitEvent(b->caller, b->join_ev, 0));
         return DMANN_OK;
     }
     if (!b->exec || b->graph_parts != (const void *)dev_parts || b->graph_n != n) {
@@ -306,6 +320,15 @@ int dmann_batch_render_device(dmann_batch *b, const dmann_particle *dev_parts, i
     }
     BCHK(b, hipGraphLaunch(b->exec, b->stream));
     b->replays++;
+    BCHK(b, hipEventRecord(b->join_ev, b->stream));
+    BCHK(b, hipStreamWaitEvent(b->caller, b->join_ev, 0));
+    return DMANN_OK;
+}
+
+int dmann_batch_set_stream(dmann_batch *b, void *stream)
+{
+    if (!b) return DMANN_ERR_ARG;
+    b->caller = (hipStream_t)stream;
     return DMANN_OK;
 }
 
@@ -314,6 +337,13 @@ int dmann_batch_synchronize(dmann_batch *b)
     if (!b) return DMANN_ERR_ARG;
     BCHK(b, hipSetDevice(b->device));
     BCHK(b, hipStreamSynchronize(b->stream));
+    std::vector<unsigned long long> h((size_t)b->nmaps * kNumStats);
+    BCHK(b, hipMemcpy(h.data(), b->d_stats, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int m = 0; m < b->nmaps; m++)
+        if (h[(size_t)m * kNumStats + kStatBad])
+            return bfail(b, DMANN_ERR_DOMAIN,
+                         "map %d: %llu particle(s) with undefined direction (d == 0 or |cos theta| > 1)", m,
+                         h[(size_t)m * kNumStats + kStatBad]);
     return DMANN_OK;
 }
 

@@ -43,6 +43,7 @@ struct dmann_ctx {
     int cus = 0;
     bool timing = false;
     std::vector<hipEvent_t> events; // 5 per timed render call
+    hipEvent_t order_ev = nullptr;  // orders a new stream after the old one
     // halo cores of the ANN_CORE_NOSOM plug-in (device copies)
     float4 *d_halos = nullptr;
     int *d_cell_start = nullptr;
@@ -89,28 +90,38 @@ static void free_group(dmann_ctx *c)
     c->gc.tab_i = nullptr;
 }
 
-// Sort-cell buffers for grouped rendering (k > 0); k == 0 frees them.
+// Sort-cell buffers for grouped rendering (k > 0); k == 0 frees them.  On
+// an allocation failure everything is freed and the context falls back to
+// the per-particle kernel (k = 0), so it never holds k > 0 with missing
+// buffers.
 static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
 {
     free_group(c);
-    c->gc.k = k;
+    c->gc.k = 0;
     c->gc.nside_c = nside_c;
-    if (k <= 0 || c->acc == kAccF64) {
-        c->gc.k = 0;
-        return hipSuccess;
-    }
+    if (k <= 0 || c->acc == kAccF64) return hipSuccess;
     const int64_t cap = c->max_parts ? c->max_parts : 1;
     const int64_t nbins = 12LL * nside_c * nside_c * kSizeBuckets;
-    hipError_t e;
-    if ((e = hipMalloc(&c->gc.keys, cap * sizeof(unsigned))) != hipSuccess) return e;
-    if ((e = hipMalloc(&c->gc.idx, cap * sizeof(unsigned))) != hipSuccess) return e;
-    if ((e = hipMalloc(&c->gc.hist, nbins * sizeof(unsigned))) != hipSuccess) return e;
-    if ((e = hipMalloc(&c->gc.sums, ((nbins + 1023) / 1024) * sizeof(unsigned))) != hipSuccess) return e;
-    if ((e = hipMalloc(&c->gc.work, sizeof(unsigned))) != hipSuccess) return e;
     const size_t rows = (size_t)c->gc.group_blocks * (size_t)k * 64;
-    if ((e = hipMalloc(&c->gc.tab_f, rows * sizeof(float4))) != hipSuccess) return e;
-    if ((e = hipMalloc(&c->gc.tab_i, rows * sizeof(int4))) != hipSuccess) return e;
+    hipError_t e;
+    if ((e = hipMalloc(&c->gc.keys, cap * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.idx, cap * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.hist, nbins * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.sums, ((nbins + 1023) / 1024) * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.work, sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.tab_f, rows * sizeof(float4))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.tab_i, rows * sizeof(int4))) != hipSuccess) {
+        free_group(c);
+        return e;
+    }
+    c->gc.k = k;
     return hipSuccess;
+}
+
+// True when grouped rendering is configured with all of its buffers.
+static bool group_ready(const GroupCfg &gc)
+{
+    return gc.k > 0 && gc.keys && gc.idx && gc.hist && gc.sums && gc.work && gc.tab_f && gc.tab_i;
 }
 
 static void free_halos(dmann_ctx *c)
@@ -346,7 +357,15 @@ int dmann_set_halo_cores(dmann_ctx *c, const float *halos, int64_t n)
 int dmann_set_stream(dmann_ctx *c, void *stream)
 {
     if (!c) return DMANN_ERR_ARG;
-    c->stream = (hipStream_t)stream;
+    const hipStream_t s = (hipStream_t)stream;
+    if (s == c->stream) return DMANN_OK;
+    // work queued on the old stream (map clears, renders, reads) comes
+    // before anything enqueued on the new one
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->order_ev) HIPCHK(c, hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming));
+    HIPCHK(c, hipEventRecord(c->order_ev, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(s, c->order_ev, 0));
+    c->stream = s;
     return DMANN_OK;
 }
 
@@ -371,6 +390,7 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
         }
         ev = &c->events[c->events.size() - 5];
     }
+    if (c->gc.k > 0 && !group_ready(c->gc)) return fail(c, DMANN_ERR_HIP, "grouping buffers are missing");
     HIPCHK(c, launch_render(c->acc, reinterpret_cast<const Particle *>(dev_parts), n, c->P, c->d_recs,
                             c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev, &c->gc));
     return DMANN_OK;
@@ -426,6 +446,7 @@ int dmann_render(dmann_ctx *c, const dmann_particle *host_parts, int64_t n)
         return fail(c, DMANN_ERR_CAPACITY, "%lld particles exceed the context capacity %lld", (long long)n,
                     (long long)c->max_parts);
     HIPCHK(c, hipSetDevice(c->device));
+    if (c->gc.k > 0 && !group_ready(c->gc)) return fail(c, DMANN_ERR_HIP, "grouping buffers are missing");
     if (!c->d_parts) HIPCHK(c, hipMalloc(&c->d_parts, c->max_parts * sizeof(Particle)));
     HIPCHK(c, hipMemcpyAsync(c->d_parts, host_parts, n * sizeof(Particle), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_render(c->acc, c->d_parts, n, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg,
@@ -501,6 +522,7 @@ void dmann_destroy(dmann_ctx *c)
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     drop_events(c);
+    if (c->order_ev) (void)hipEventDestroy(c->order_ev);
     free_all(c);
     delete c;
 }

@@ -611,12 +611,13 @@ __global__ __launch_bounds__(256) void k_batch(const Particle *__restrict__ part
         const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
         const DevParams P = Ps[m];
         const MapRef map = maps[m];
-        unsigned long long c_kept = 0, c_sub = 0, c_fb = 0, c_ext = 0, c_contrib = 0;
+        unsigned long long c_kept = 0, c_sub = 0, c_fb = 0, c_ext = 0, c_contrib = 0, c_bad = 0;
         for (int64_t i = i0; i < i1; ++i) {
             const Particle q = parts[i];
             DiscRec rec;
             bool bad;
             const int cls = prepass_one(P, q, rec, bad);
+            if (bad) c_bad++;
             if (cls == kClsSkip) continue;
             c_kept++;
             if (cls == kClsPoint) {
@@ -640,6 +641,7 @@ __global__ __launch_bounds__(256) void k_batch(const Particle *__restrict__ part
             if (c_fb) atomicAdd(&st[kStatFallback], c_fb);
             if (c_ext) atomicAdd(&st[kStatExt], c_ext);
             if (c_contrib) atomicAdd(&st[kStatContrib], c_contrib);
+            if (c_bad) atomicAdd(&st[kStatBad], c_bad);
         }
     }
 }

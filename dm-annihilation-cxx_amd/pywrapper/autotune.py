@@ -1,7 +1,8 @@
 """Autotuner for the projector's launch shapes and size-class threshold.
 
-Times candidate settings on a particle sample with HIP events (through
-torch's stream) and keeps the fastest.  Single map: persistent workgroups
+Times candidate settings on a particle sample with HIP events on torch's
+current stream -- projectors run on it, batches fork from and join back
+into it, so the events bracket the GPU work -- and keeps the fastest.  Single map: persistent workgroups
 per CU of the two footprint kernels, the footprint size that sends a
 particle to the workgroup kernel and, optionally, the group size of the
 grouped medium-class kernel.  Batch: the per-map pipeline, or particles per

@@ -73,7 +73,7 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_reset", "dmann_get_stats", "dmann_npix", "dmann_destroy",
            "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint", "dmann_host_ring_check",
            "dmann_timing", "dmann_adjoint_device", "dmann_last_error", "dmann_set_tuning", "dmann_set_grouping",
-           "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device",
+           "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device", "dmann_batch_set_stream",
            "dmann_batch_synchronize", "dmann_batch_npix", "dmann_batch_map_device", "dmann_batch_get_map",
            "dmann_batch_get_stats", "dmann_batch_graph_replays", "dmann_batch_destroy",
            "dmann_batch_last_error", "dmann_read_particles", "dmann_set_halo_cores", "dmann_read_halo_file")
@@ -132,6 +132,7 @@ def load_library():
     L.dmann_batch_create.argtypes = [C.POINTER(vp), vp, C.c_int, C.c_int]
     L.dmann_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.dmann_batch_render_device.argtypes = [vp, vp, i64, C.c_int]
+    L.dmann_batch_set_stream.argtypes = [vp, vp]
     L.dmann_batch_synchronize.argtypes = [vp]
     L.dmann_batch_npix.argtypes = [vp, C.c_int]
     L.dmann_batch_npix.restype = i64
@@ -240,18 +241,22 @@ class Projector:
             return
         a = _as_host_records(parts)
         n = a.shape[0]
+        self._bind_stream()
         _check(self._L.dmann_render(self._h, C.c_void_p(a.ctypes.data), int(n)), self._h)
 
     def _bind_stream(self):
-        import torch
-        s = torch.cuda.current_stream(self.device).cuda_stream
+        """Every call runs on torch's current stream of the device; the
+        library orders a newly bound stream after the previous one."""
+        s = _torch_stream(self.device)
         _check(self._L.dmann_set_stream(self._h, C.c_void_p(s)), self._h)
 
     def synchronize(self) -> None:
+        self._bind_stream()
         _check(self._L.dmann_synchronize(self._h), self._h)
 
     def get_map(self) -> np.ndarray:
         out = np.empty(self.npix, dtype=np.float64)
+        self._bind_stream()
         _check(self._L.dmann_get_map(self._h, C.c_void_p(out.ctypes.data)), self._h)
         return out
 
@@ -299,6 +304,7 @@ class Projector:
         precedes it."""
         ms = (C.c_double * 4)()
         calls = C.c_int()
+        self._bind_stream()
         _check(self._L.dmann_timing(self._h, int(bool(enable)), ms, C.byref(calls)), self._h)
         return {"prepass": ms[0], "large": ms[1], "sort": ms[2], "medium": ms[3]}, calls.value
 
@@ -309,10 +315,12 @@ class Projector:
         _check(self._L.dmann_set_halo_cores(self._h, C.c_void_p(h.ctypes.data), int(h.shape[0])), self._h)
 
     def reset(self) -> None:
+        self._bind_stream()
         _check(self._L.dmann_reset(self._h), self._h)
 
     def stats(self) -> dict:
         st = DmannStats()
+        self._bind_stream()
         _check(self._L.dmann_get_stats(self._h, C.byref(st)), self._h)
         return st.as_dict()
 
@@ -332,6 +340,18 @@ class Projector:
             self.close()
         except Exception:
             pass
+
+
+def _torch_stream(device: int) -> int:
+    """Handle of torch's current stream on `device` (0, the null stream,
+    when torch has no GPU)."""
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return 0
+    if not torch.cuda.is_available():
+        return 0
+    return int(torch.cuda.current_stream(device).cuda_stream)
 
 
 def read_halo_file(path: str) -> np.ndarray:
@@ -395,8 +415,8 @@ def host_ring_check(params: DmannParams, rec) -> int:
 # ------------------------------------------------------------------ batch
 
 class Batch:
-    """Many maps (observers, mixed nside) of one particle set, rendered in
-    one persistent launch replayed from a hipGraph."""
+    """Many maps (observers, mixed nside) of one particle set, rendered
+    together and replayed from a hipGraph (dmann_batch_*)."""
 
     def __init__(self, params_list, device: int = 0):
         L = load_library()
@@ -408,6 +428,7 @@ class Batch:
         if code != 0:
             raise DmannError(code, (L.dmann_batch_last_error(None) or b"").decode())
         self._h = h
+        self.device = device
         self.nmaps = len(params_list)
         self.npix = [int(L.dmann_batch_npix(h, i)) for i in range(self.nmaps)]
 
@@ -420,7 +441,10 @@ class Batch:
 
     def render(self, parts, use_graph: bool = True):
         """parts: (n, 12) float32 CUDA tensor, kept alive by the caller
-        while a captured graph may replay it."""
+        while a captured graph may replay it.  The render is ordered after
+        the work on torch's current stream, and work enqueued there later
+        sees the finished maps."""
+        self._chk(self._L.dmann_batch_set_stream(self._h, C.c_void_p(_torch_stream(self.device))))
         self._chk(self._L.dmann_batch_render_device(self._h, C.c_void_p(parts.data_ptr()), int(parts.shape[0]),
                                                     int(bool(use_graph))))
 
@@ -436,6 +460,21 @@ class Batch:
         st = DmannStats()
         self._chk(self._L.dmann_batch_get_stats(self._h, int(i), C.byref(st)))
         return st.as_dict()
+
+    def map_device(self, i: int):
+        """Finished map i (float64, npix, divided by dOmega) as a torch
+        tensor viewing the batch's device buffer (no copy); valid until the
+        batch is closed.  Read it on the stream the render was issued from
+        (or after synchronize())."""
+        import torch
+        ptr = C.c_void_p()
+        self._chk(self._L.dmann_batch_map_device(self._h, int(i), C.byref(ptr)))
+
+        class _View:
+            __cuda_array_interface__ = {"shape": (self.npix[i],), "typestr": "<f8",
+                                        "data": (int(ptr.value), False), "version": 2}
+
+        return torch.as_tensor(_View(), device=f"cuda:{self.device}")
 
     def replays(self) -> int:
         return int(self._L.dmann_batch_graph_replays(self._h))

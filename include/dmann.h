@@ -238,6 +238,13 @@ int dmann_batch_create(dmann_batch **b, const dmann_params *params, int nmaps, i
  * persistent workgroups per CU */
 int dmann_batch_set_tuning(dmann_batch *b, int chunk, int blocks_per_cu);
 int dmann_batch_render_device(dmann_batch *b, const dmann_particle *dev_parts, int64_t n, int use_graph);
+/* The caller's stream (a hipStream_t; NULL = the null stream, the default):
+ * each render waits for the work queued on it before reading the particles,
+ * and work queued on it after the render waits for the maps.  The render
+ * itself (and its graph) runs on the batch's own stream. */
+int dmann_batch_set_stream(dmann_batch *b, void *stream);
+/* Waits for the batch's work.  Reports DMANN_ERR_DOMAIN when a kept particle
+ * had an undefined direction for some map's observer. */
 int dmann_batch_synchronize(dmann_batch *b);
 int64_t dmann_batch_npix(const dmann_batch *b, int i);
 /* device pointer of finished map i (npix doubles, / dOmega), valid until destroy */

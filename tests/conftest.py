@@ -57,4 +57,20 @@ def rel_err_report(gpu_map, ref_map, floor=1e-6):
     below = float(np.abs(got[~sel] - ref[~sel]).max() / mx) if (~sel).any() and mx > 0 else 0.0
     tot_ref = ref.sum()
     tot_rel = float(abs(got.sum() - tot_ref) / abs(tot_ref)) if tot_ref != 0 else float(abs(got.sum()))
-    return {"max_rel": max_rel, "total_rel": tot_rel, "below_floor_abs": below, "n_sel": int(sel.sum())}
+    # pixels the reference leaves empty (no footprint covers them) must stay
+    # exactly empty: a deposit at a wrong address shows up here even when the
+    # contribution counts and the total still match
+    zero_mismatch = int(np.count_nonzero((ref == 0) & (got != 0)))
+    return {"max_rel": max_rel, "total_rel": tot_rel, "below_floor_abs": below, "n_sel": int(sel.sum()),
+            "floor": floor, "zero_mismatch": zero_mismatch}
+
+
+def assert_map(rep, tol, ttol=None):
+    """The map bar: per-pixel relative error <= tol above floor*max, the same
+    relative bar in absolute terms (tol * floor * max) below it, no deposit
+    into a pixel the reference leaves empty, and (ttol) the total."""
+    assert rep["max_rel"] <= tol, rep
+    assert rep["below_floor_abs"] <= tol * rep["floor"], rep
+    assert rep["zero_mismatch"] == 0, rep
+    if ttol is not None:
+        assert rep["total_rel"] <= ttol, rep

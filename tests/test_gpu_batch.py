@@ -10,7 +10,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import rel_err_report
+from conftest import assert_map, rel_err_report
 
 pytestmark = pytest.mark.gpu
 
@@ -50,9 +50,9 @@ def test_batch_matches_single_observer(pw, gpu, precision, tol, chunk):
         for key in ("kept", "subpixel", "fallback", "extended", "contributions"):
             assert stats[k][key] == st[key], (k, key)
         rep = rel_err_report(maps[k], ref)
-        assert rep["max_rel"] <= tol, (k, rep)
+        assert_map(rep, tol)
         rep2 = rel_err_report(again[k], maps[k])
-        assert rep2["max_rel"] <= tol, (k, rep2)
+        assert_map(rep2, tol)
     torch.cuda.synchronize()
 
 
@@ -71,9 +71,16 @@ def test_autotuner_runs(pw, gpu):
     with pw.Projector(p, parts.shape[0]) as pr:
         pr.render(parts)
         m0 = pr.get_map()
-    assert rel_err_report(m1, m0)["max_rel"] <= 2e-6
+    assert_map(rel_err_report(m1, m0), 2e-6)
     plist = [_observer_params(pw, k, 4, 64, "fp32") for k in range(4)]
     with pw.Batch(plist) as b:
         best, res = pw.autotune.tune_batch(b, parts, chunks=(0, 16, 64), blocks=(8,))
         assert len(res) == 3 and best["chunk"] in (0, 16, 64)
+        # the timer sees the batch's GPU work (not only the host enqueue):
+        # 8x the particles takes clearly longer
+        more = pw.synth.make("nfw_sub100_1e7", n=160000).to(gpu)
+        b.set_tuning(0, 8)
+        t_small = pw.autotune._time(lambda: b.render(parts, use_graph=False), 3)
+        t_big = pw.autotune._time(lambda: b.render(more, use_graph=False), 3)
+        assert t_big > 2.0 * t_small, (t_small, t_big)
     torch.cuda.synchronize()

@@ -9,7 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import PKG, rel_err_report
+from conftest import PKG, assert_map, rel_err_report
 
 pytestmark = pytest.mark.gpu
 EXE = os.path.join(PKG, "annmap_hip")
@@ -43,7 +43,7 @@ def test_annmap_hip_end_to_end(pw, orc, gpu, tmp_path):
     ref, st = orc.genmap(parts, cfg, nthreads=4)
     rep = rel_err_report(got, ref)
     print(rep)
-    assert rep["max_rel"] <= 1e-5 and rep["total_rel"] <= 1e-6
+    assert_map(rep, 1e-5, 1e-6)
     # a second run overwrites the existing file
     r = subprocess.run([EXE, "-info", ini], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "File exists" in r.stdout

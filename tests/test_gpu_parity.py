@@ -14,7 +14,7 @@ the exact footprint pixel counts) must match the oracle exactly.
 import numpy as np
 import pytest
 
-from conftest import rel_err_report
+from conftest import assert_map, rel_err_report
 
 pytestmark = pytest.mark.gpu
 
@@ -69,8 +69,7 @@ def test_nfw_1e5_nside64(pw, orc, gpu, nfw_1e5, oracle_nfw_1e5, precision):
     rep = rel_err_report(m, ref)
     print(rep)
     tol, ttol = TOL[precision]
-    assert rep["max_rel"] <= tol
-    assert rep["total_rel"] <= ttol
+    assert_map(rep, tol, ttol)
 
 
 @pytest.mark.parametrize("nside", [256, 512])
@@ -86,8 +85,7 @@ def test_subhalo_sample_high_nside(pw, orc, gpu, precision, nside):
     rep = rel_err_report(m, ref)
     print(rep)
     tol, ttol = TOL[precision]
-    assert rep["max_rel"] <= tol
-    assert rep["total_rel"] <= ttol
+    assert_map(rep, tol, ttol)
 
 
 def test_chunked_accumulation_matches_single_call(pw, gpu, nfw_1e5):
@@ -97,7 +95,7 @@ def test_chunked_accumulation_matches_single_call(pw, gpu, nfw_1e5):
     m4, st4 = _run_gpu(pw, nfw_1e5, 64, "fp64", chunks=7)
     assert st1 == st4
     rep = rel_err_report(m4, m1)
-    assert rep["max_rel"] <= 1e-12
+    assert_map(rep, 1e-12)
 
 
 @pytest.mark.parametrize("k", [0, 1, 16, 64])
@@ -118,7 +116,7 @@ def test_grouping_sizes(pw, orc, gpu, precision, k):
     rep = rel_err_report(m, ref)
     print(rep)
     tol, ttol = TOL[precision]
-    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+    assert_map(rep, tol, ttol)
 
 
 def _sky_spread(pw, n, seed, rad=0.06):
@@ -158,7 +156,7 @@ def test_headline_model_sample_high_nside(pw, orc, gpu, precision, nside):
     rep = rel_err_report(m, ref)
     print(rep, st["large"])
     tol, ttol = TOL[precision]
-    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+    assert_map(rep, tol, ttol)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "compensated", "fp32"])
@@ -182,7 +180,7 @@ def test_sky_covering_size_classes(pw, orc, gpu, precision):
     rep = rel_err_report(m, ref)
     print(rep, st["large"])
     tol, ttol = TOL[precision]
-    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+    assert_map(rep, tol, ttol)
 
 
 @pytest.mark.parametrize("precision", ["compensated", "fp32"])
@@ -205,7 +203,7 @@ def test_grouping_wide_windows(pw, orc, gpu, precision):
     rep = rel_err_report(m, ref)
     print(rep)
     tol, ttol = TOL[precision]
-    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+    assert_map(rep, tol, ttol)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "compensated"])
@@ -234,7 +232,7 @@ def test_flux_plugins(pw, orc, gpu, precision):
         _check_counts(st, ost)
         rep = rel_err_report(m, ref)
         print(kind, rep)
-        assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol, kind
+        assert_map(rep, tol, ttol)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "compensated"])
@@ -261,7 +259,7 @@ def test_cuda_cutoff(pw, orc, gpu, precision):
     rep = rel_err_report(m, ref)
     print(rep)
     tol, ttol = TOL[precision]
-    assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+    assert_map(rep, tol, ttol)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "compensated"])
@@ -320,4 +318,4 @@ def test_halo_core_plugin(pw, orc, gpu, precision):
                 # one-ulp difference between device and host sin/cos in the
                 # pixel vectors to ~1e-9 of a weight
                 tol = 1e-8
-            assert rep["max_rel"] <= tol and rep["total_rel"] <= ttol
+            assert_map(rep, tol, ttol)

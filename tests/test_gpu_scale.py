@@ -15,7 +15,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import rel_err_report
+from conftest import assert_map, rel_err_report
 
 pytestmark = pytest.mark.gpu
 
@@ -47,7 +47,7 @@ def test_sample_5e6_nside2048_compensated_vs_fp64(pw, gpu, k):
         assert st[key] == st_ref[key], key
     rep = rel_err_report(got, ref)
     print(rep, st, flush=True)
-    assert rep["max_rel"] <= 1e-5 and rep["total_rel"] <= 1e-6
+    assert_map(rep, 1e-5, 1e-6)
     # invariant of src/genmap.cpp:161-172: sum(map) dOmega = sum of the
     # kept particles' fluxes (FP64 from the same inputs)
     p = pw.params_for_observer(2048, "fp64")
