@@ -158,6 +158,8 @@ static void free_all(dmann_ctx *c)
 }
 
 namespace dmann {
+void set_global_error(const char *msg) { snprintf(g_create_msg, sizeof g_create_msg, "%s", msg); }
+
 void fill_devparams(const dmann_params *p, DevParams &P, int large_pix)
 {
     P = DevParams{};

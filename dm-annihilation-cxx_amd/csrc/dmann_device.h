@@ -105,6 +105,8 @@ __device__ __forceinline__ void acc_add(const MapRef &m, int addr, float v32, do
         float bb = s - old;
         float err = (old - (s - bb)) + (v32 - bb);
         if (err != 0.0f) unsafeAtomicAdd(m.lo + addr, err);
+    } else if constexpr (ACC == kAccW64) {
+        unsafeAtomicAdd(m.f64 + addr, (double)v32);
     } else {
         unsafeAtomicAdd(m.f64 + addr, v64);
     }
@@ -123,6 +125,8 @@ __device__ __forceinline__ void acc_point(const MapRef &m, int addr, double flux
         float bb = s - old;
         float err = (old - (s - bb)) + (f - bb) + rem;
         if (err != 0.0f) unsafeAtomicAdd(m.lo + addr, err);
+    } else if constexpr (ACC == kAccW64) {
+        unsafeAtomicAdd(m.f64 + addr, flux);
     } else {
         acc_add<ACC>(m, addr, (float)flux, flux);
     }

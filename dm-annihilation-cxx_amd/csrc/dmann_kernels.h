@@ -17,7 +17,10 @@ struct Particle {
 };
 static_assert(sizeof(Particle) == 48, "DMParticle is 12 floats");
 
-enum { kAccF32 = 0, kAccComp = 1, kAccF64 = 2 };
+// Accumulation forms: FP32 weights into an FP32 map, into a two-float map,
+// FP64 weights into an FP64 map, and FP32 weights into an FP64 map (FP64
+// atomics; the allocation-free entry point accumulates into its output).
+enum { kAccF32 = 0, kAccComp = 1, kAccF64 = 2, kAccW64 = 3 };
 enum { kClsSkip = 0, kClsPoint = 1, kClsMedium = 2, kClsLarge = 3 };
 enum {
     kStatKept = 0,
@@ -69,6 +72,9 @@ struct DevParams {
 
 // Device constants of one map from the C-ABI parameters.
 void fill_devparams(const dmann_params *p, DevParams &P, int large_pix);
+
+// Message of a failed call without a context (dmann_last_error(NULL)).
+void set_global_error(const char *msg);
 
 // Accumulation targets of one map.
 struct MapRef {

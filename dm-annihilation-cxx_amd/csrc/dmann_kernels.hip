@@ -705,7 +705,7 @@ __global__ void k_finalize(MapRef map, int64_t npix, double dOmega, double *__re
         double v;
         if constexpr (ACC == kAccF32) v = (double)map.f32[i];
         else if constexpr (ACC == kAccComp) v = (double)map.f32[i] + (double)map.lo[i];
-        else v = map.f64[i];
+        else v = map.f64[i]; // FP64 and W64 maps; in place when out == map.f64
         out[i] = v / dOmega;
     }
 }
@@ -726,7 +726,7 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
     if (ev) (void)hipEventRecord(ev[1], st);
     k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, P.footprint_pad, map, stats);
     if (ev) (void)hipEventRecord(ev[2], st);
-    if (ACC != kAccF64 && gc && gc->k > 0) {
+    if ((ACC == kAccF32 || ACC == kAccComp) && gc && gc->k > 0) {
         const hipError_t e =
             launch_group(ACC, recs, counts, P.nside, P.footprint_pad, *gc, map, stats, st, ev ? ev[3] : nullptr);
         if (e != hipSuccess) return e;
@@ -745,6 +745,7 @@ hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevP
     switch (acc) {
     case kAccF32: return launch_render_t<kAccF32>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
     case kAccComp: return launch_render_t<kAccComp>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
+    case kAccW64: return launch_render_t<kAccW64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, nullptr);
     default: return launch_render_t<kAccF64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
     }
 }
@@ -763,7 +764,7 @@ hipError_t launch_finalize(int acc, MapRef map, int64_t npix, double dOmega, dou
 hipError_t launch_clear(int acc, MapRef map, int64_t npix, unsigned long long *stats, hipStream_t st)
 {
     const int blocks = (int)((npix + 255) / 256 < 8192 ? (npix + 255) / 256 : 8192);
-    if (acc == kAccF64) k_fill_f64<<<blocks, 256, 0, st>>>(map.f64, npix);
+    if (acc == kAccF64 || acc == kAccW64) k_fill_f64<<<blocks, 256, 0, st>>>(map.f64, npix);
     else k_fill_f32<<<blocks, 256, 0, st>>>(map.f32, npix);
     if (acc == kAccComp) k_fill_f32<<<blocks, 256, 0, st>>>(map.lo, npix);
     k_fill_u64<<<1, 64, 0, st>>>(stats, kNumStats);

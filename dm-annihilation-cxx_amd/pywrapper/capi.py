@@ -76,7 +76,8 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device", "dmann_batch_set_stream",
            "dmann_batch_synchronize", "dmann_batch_npix", "dmann_batch_map_device", "dmann_batch_get_map",
            "dmann_batch_get_stats", "dmann_batch_graph_replays", "dmann_batch_destroy",
-           "dmann_batch_last_error", "dmann_read_particles", "dmann_set_halo_cores", "dmann_read_halo_file")
+           "dmann_batch_last_error", "dmann_read_particles", "dmann_set_halo_cores", "dmann_read_halo_file",
+           "dmann_render_map_noalloc")
 
 _lib = None
 
@@ -145,6 +146,7 @@ def load_library():
     L.dmann_batch_destroy.restype = None
     L.dmann_batch_last_error.argtypes = [vp]
     L.dmann_batch_last_error.restype = C.c_char_p
+    L.dmann_render_map_noalloc.argtypes = [C.POINTER(DmannParams), vp, i64, vp, vp, C.POINTER(DmannStats)]
     _lib = L
     return L
 
@@ -495,6 +497,29 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ no-alloc
+
+def render_map_noalloc(params: DmannParams, parts, out, stats: bool = False):
+    """Allocation-free render (dmann_render_map_noalloc): the finished FP64
+    map of `parts` ((n, 12) float32 CUDA tensor) into `out` (float64 CUDA
+    tensor of npix), on torch's current stream; no device memory is
+    allocated.  stats=True waits and returns the statistics dict."""
+    import torch
+    npix = 12 * params.nside * params.nside
+    if not (out.is_cuda and out.dtype == torch.float64 and out.numel() == npix and out.is_contiguous()):
+        raise ValueError("out must be a contiguous float64 CUDA tensor of npix elements")
+    if not (parts.is_cuda and parts.dtype == torch.float32 and parts.dim() == 2 and parts.shape[1] == 12
+            and parts.is_contiguous()):
+        raise ValueError("parts must be a contiguous (n, 12) float32 CUDA tensor")
+    L = load_library()
+    st = DmannStats() if stats else None
+    code = L.dmann_render_map_noalloc(C.byref(params), C.c_void_p(parts.data_ptr()), int(parts.shape[0]),
+                                      C.c_void_p(out.data_ptr()), C.c_void_p(_torch_stream(out.device.index or 0)),
+                                      C.byref(st) if st is not None else None)
+    _check(code)
+    return st.as_dict() if st is not None else None
 
 
 # ------------------------------------------------------------------ FITS

@@ -308,8 +308,30 @@ int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int
  * reference. */
 int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q);
 
-/* Message of the last failed call on ctx (or of the last failed
- * dmann_create when ctx is NULL). */
+/* Allocation-free render.  No counterpart in the reference: for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Renders n particles
+ * (device memory) for the map constants *p into dev_map (npix = 12 nside^2
+ * doubles of device memory, the caller's): the map is cleared first, and on
+ * return holds the finished FP64 RING map divided by dOmega, as
+ * dmann_get_map_device gives it.  Accumulation is FP64, in dev_map itself
+ * (FP32 weights for the FP32 and compensated precisions, FP64 weights for
+ * FP64); the sort-free per-particle kernels are used.  No device memory is
+ * allocated or freed: the per-chunk particle records and the counters live
+ * in a fixed 48 MiB + 256 B device array that each GPU holds from its first
+ * call until the process ends; particles pass through it in chunks of 2^20.
+ * Runs on the current device, asynchronously on `stream` (a hipStream_t,
+ * NULL = the null stream).  Calls that overlap on one GPU race on that
+ * array: serialise them (one stream, or events).  With host_stats != NULL
+ * the call waits for the render, fills *host_stats and returns
+ * DMANN_ERR_DOMAIN when a kept particle had an undefined direction; with
+ * NULL it only enqueues (such particles are skipped).  The halo-core flux
+ * plug-in is not available here (DMANN_ERR_ARG).  Errors: dmann_last_error(NULL). */
+int dmann_render_map_noalloc(const dmann_params *p, const dmann_particle *dev_parts, int64_t n, double *dev_map,
+                             void *stream, dmann_stats *host_stats);
+
+/* Message of the last failed call on ctx (or, when ctx is NULL, of the last
+ * failed dmann_create or dmann_render_map_noalloc). */
 const char *dmann_last_error(const dmann_ctx *ctx);
 
 #ifdef __cplusplus
