@@ -511,7 +511,7 @@ DMANN_HD float ring_term_weight(float x, float e0, float e1, float e2, float K4)
 
 // Pixels per segment of the grouped kernel's ring walks.
 #ifndef DMANN_GROUP_SEG
-#define DMANN_GROUP_SEG 8
+#define DMANN_GROUP_SEG 7
 #endif
 constexpr int kSeg = DMANN_GROUP_SEG;
 

@@ -7,7 +7,8 @@
 // the bottleneck.  Here the medium-class particles are first sorted by a
 // coarse sky cell (counting sort), then one wavefront takes K consecutive
 // particles (a group of neighbours):
-//   1. per particle, the normalisation pass (FP32 weights, FP64 sum);
+//   1. the normalisation pass (FP32 weights, FP64 sums): every (particle,
+//      ring) row of the group on a lane of its own, 64 rows at a time;
 //   2. the group's window -- the rings irmin..irmax of all members and the
 //      union of their longitude ranges -- is walked in bands of rings that fit
 //      the wave's LDS buffer; each member deposits its band pixels with a
@@ -30,15 +31,12 @@
 
 namespace dmann {
 
-// LDS budget of one wave: 20 KB, so 8 single-wave blocks fit a 160 KB CU.
+// LDS budget of one wave: 20 KB, so 8 single-wave blocks fit a 160 KB CU
+// (the window takes what the tables leave, in whole kSeg blocks).
 #ifndef DMANN_GROUP_CAP
-#define DMANN_GROUP_CAP 3704
+#define DMANN_GROUP_CAP ((15280 * kSeg) / (4 * kSeg + 1) / kSeg * kSeg)
 #endif
 constexpr int kGroupCapF32 = DMANN_GROUP_CAP; // FP32 window slots
-// one-table path for groups whose window fits one band
-#ifndef DMANN_GROUP_ONEBAND
-#define DMANN_GROUP_ONEBAND 1
-#endif
 
 struct GroupLDS {
     // ring table of up to 64 rings of one member
@@ -148,28 +146,6 @@ __device__ __forceinline__ int ring_table_fill(const RingLite &rl, bool on, int 
     return NS;
 }
 
-// Sets up the ring table for rings iz0 .. iz0+n-1 (n <= 64) of a member:
-// lane q computes ring iz0 + q (and stores its terms to cache[q] when a
-// cache row is given).  Returns the number of segments.
-__device__ __forceinline__ int ring_table(const PGeo &g, const RingGeo &G, int iz0, int n, GroupLDS &S, int lane,
-                                          int band_ring0, float4 *cache_f = nullptr, int4 *cache_i = nullptr)
-{
-    const bool on = lane < n;
-    RingLite rl;
-    rl.cnt = 0;
-    if (on) {
-        rl = ring_lite(g, G, iz0 + lane);
-        if (cache_f) {
-            float4 f;
-            int4 i;
-            ring_pack(rl, f, i);
-            cache_f[lane] = f;
-            cache_i[lane] = i;
-        }
-    }
-    return ring_table_fill(rl, on, iz0, S, lane, band_ring0);
-}
-
 // Deposits pixels t0 .. t0+len-1 of table ring q straight into the map
 // (global atomics), for bands whose window does not fit the LDS buffer.
 template <int ACC, int PV>
@@ -260,6 +236,41 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
         }
     }
     return total + (double)part;
+}
+
+// Sum of the weights of one table row held in a lane's registers: pixels
+// t = 0 .. cnt-1 at longitude offset x_t = offm + (t - kc_rel) step, four at
+// a time from x of the first (two roundings, as in the segment walks), the
+// last 0-3 masked.  FP32 partial sums of at most 16 weights go to the FP64
+// total.  The trip count is the lane's own: rows of different lengths
+// share the wavefront.
+template <int PV>
+__device__ __forceinline__ double row_sum(const RingLite &rl, float K4)
+{
+    const int n4 = rl.cnt >> 2;
+    double acc = 0.0;
+    float part = 0.0f;
+    for (int c = 0; c < n4; ++c) {
+        const float xb = fmaf((float)(4 * c - rl.kc_rel), rl.step, rl.offm);
+        const float w0 = ring_term_weight<PV>(xb, rl.e0, rl.e1, rl.e2, K4);
+        const float w1 = ring_term_weight<PV>(fmaf(1.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
+        const float w2 = ring_term_weight<PV>(fmaf(2.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
+        const float w3 = ring_term_weight<PV>(fmaf(3.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
+        part += (w0 + w1) + (w2 + w3);
+        if ((c & 3) == 3) {
+            acc += (double)part;
+            part = 0.0f;
+        }
+    }
+    const int t0 = 4 * n4;
+    const float lenf = (float)(rl.cnt - t0);
+    const float xb = fmaf((float)(t0 - rl.kc_rel), rl.step, rl.offm);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float w = ring_term_weight<PV>(k == 0 ? xb : fmaf((float)k, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
+        part = fmaf(w, seg_mask(lenf, k), part);
+    }
+    return acc + (double)part;
 }
 
 // walk_segments with the particle's series lengths (fast_variant).
@@ -433,7 +444,6 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
         if (grp >= ngroups) break;
         const unsigned m0 = grp * (unsigned)K;
         const int nm = (int)((m0 + (unsigned)K < count ? m0 + (unsigned)K : count) - m0);
-        // ---- phase 1: normalisation of every member
         // member geometry, one member per lane
         float my_scale = 0.0f, my_K4 = 0.0f;
         int my_pv = 2, my_p0 = 0;
@@ -455,90 +465,93 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
             my_flux = mg.flux;
             my_p0 = mg.p0;
         }
-#if DMANN_GROUP_ONEBAND
-        // ---- a group whose whole window fits one band: each member is
-        // normalised and deposited in turn from one ring table (members
-        // span at most 64 rings then), with no cached ring terms
-        {
-            const bool gon = lane < nm && my_hi >= my_lo;
-            const int G0 = wave_min_i(gon ? my_lo : 0x7fffffff);
-            const int G1 = wave_max_i(gon ? my_hi : -0x7fffffff);
-            if (G0 <= G1 && G1 - G0 < 64) {
-                const Window gw = group_window(gon, my_phi, my_dphi);
-                const Band bd = band_setup(S, lane, nside, G0, G1, gw.phic, gw.wlo, gw.whi, gw.full);
-                if (!bd.direct && bd.BH > G1 - G0) {
-                    for (int j = 0; j < nm; ++j) {
-                        PGeo g;
-                        g.phi = wave_read(my_phi, j);
-                        g.z0 = wave_read(my_z0, j);
-                        g.s0 = wave_read(my_s0, j);
-                        g.xa = wave_read(my_xa, j);
-                        g.cosr = wave_read(my_cosr, j);
-                        g.K4 = wave_read(my_K4, j);
-                        g.pv = wave_read(my_pv, j);
-                        g.irmin = wave_read(my_lo, j);
-                        g.irmax = wave_read(my_hi, j);
-                        const int n = g.irmax >= g.irmin ? g.irmax - g.irmin + 1 : 0;
-                        const int NS = ring_table(g, G, g.irmin, n, S, lane, G0);
-                        const int T = wave_reduce_full<DppAdd>(lane < n ? S.ri[lane].y : 0, 0);
-                        const double norm =
-                            wave_reduce_full<DppAdd>(walk_pv<0, ACC, false>(g.pv, S, NS, lane, g.K4, 0.0f, map), 0.0);
-                        if (T < 2) {
-                            if (lane == j) acc_point<ACC>(map, my_p0, my_flux);
-                            n_fb++;
-                        } else {
-                            n_ext++;
-                            n_contrib += (unsigned long long)T;
-                            const float sc = (float)(wave_read(my_flux, j) / norm);
-                            walk_pv<1, ACC, true>(g.pv, S, NS, lane, g.K4, sc, map);
-                        }
-                        wave_sync();
+        // ---- phase 1: normalisation.  Every (member, ring) row of the group
+        // gets a lane of its own, 64 rows per pass, in member order: the lane
+        // computes the row's column range and weight terms (ring_lite),
+        // keeps them in registers for the walk over the row's pixels, and
+        // stores them to the wave's ring-term cache (rows below kCacheRows)
+        // for the deposit bands.  Row sums are then added up per member.
+        const int nrow = (lane < nm && my_hi >= my_lo) ? my_hi - my_lo + 1 : 0;
+        const int rincl = wave_scan_full<DppAdd>(nrow, 0);
+        const int rstart = rincl - nrow; // member's first row in the group
+        const int NR = wave_read(rincl, 63);
+        const size_t cache0 = (size_t)blockIdx.x * (size_t)K * 64;
+        const int cache_rows = K * 64;
+        double my_norm = 0.0;
+        int my_T = 0;
+        int carry = 0; // member of the last row of the previous pass
+        // member marks of one pass (the ring table is not in use here)
+        int *mark = reinterpret_cast<int *>(S.ri);
+        for (int p0 = 0; p0 < NR; p0 += 64) {
+            const int g = p0 + lane;
+            const bool valid = g < NR;
+            mark[lane] = -1;
+            wave_sync();
+            if (nrow > 0 && rstart >= p0 && rstart < p0 + 64) mark[rstart - p0] = lane;
+            wave_sync();
+            // member of row g: the last member starting at or before it
+            int mj = wave_scan_full<DppMax>(mark[lane], -1);
+            if (mj < 0) mj = carry;
+            carry = wave_read(mj, 63);
+            PGeo gm;
+            gm.phi = __shfl(my_phi, mj, 64);
+            gm.z0 = __shfl(my_z0, mj, 64);
+            gm.s0 = __shfl(my_s0, mj, 64);
+            gm.xa = __shfl(my_xa, mj, 64);
+            gm.cosr = __shfl(my_cosr, mj, 64);
+            gm.K4 = __shfl(my_K4, mj, 64);
+            gm.pv = __shfl(my_pv, mj, 64);
+            const int lo = __shfl(my_lo, mj, 64);
+            const int rs0 = __shfl(rstart, mj, 64);
+            RingLite rl;
+            rl.cnt = 0;
+            double rowsum = 0.0;
+            if (valid) {
+                rl = ring_lite(gm, G, lo + (g - rs0));
+                if (g < cache_rows) {
+                    float4 f;
+                    int4 i;
+                    ring_pack(rl, f, i);
+                    tab_f[cache0 + (size_t)g] = f;
+                    tab_i[cache0 + (size_t)g] = i;
+                }
+                if (rl.cnt > 0) {
+                    switch (gm.pv) {
+                    case 0: rowsum = row_sum<0>(rl, gm.K4); break;
+                    case 1: rowsum = row_sum<1>(rl, gm.K4); break;
+                    default: rowsum = row_sum<2>(rl, gm.K4); break;
                     }
-                    flush_band<ACC>(S, bd.Sb, lane, map, npix);
-                    wave_sync();
-                    continue;
+                }
+            }
+            // per-member totals of this pass's rows
+            const int mfirst = wave_read(mj, 0);
+            const int mlast = wave_read(mj, NR - 1 - p0 < 63 ? NR - 1 - p0 : 63);
+            for (int m = mfirst; m <= mlast; ++m) {
+                const bool in = valid && mj == m;
+                const double sm = wave_reduce_full<DppAdd>(in ? rowsum : 0.0, 0.0);
+                const int tm = wave_reduce_full<DppAdd>(in ? rl.cnt : 0, 0);
+                if (lane == m) {
+                    my_norm += sm;
+                    my_T += tm;
                 }
             }
         }
-#endif
-        for (int j = 0; j < nm; ++j) {
-            PGeo g;
-            g.phi = wave_read(my_phi, j);
-            g.z0 = wave_read(my_z0, j);
-            g.s0 = wave_read(my_s0, j);
-            g.xa = wave_read(my_xa, j);
-            g.cosr = wave_read(my_cosr, j);
-            g.K4 = wave_read(my_K4, j);
-            g.pv = wave_read(my_pv, j);
-            g.irmin = wave_read(my_lo, j);
-            g.irmax = wave_read(my_hi, j);
-            // medium-class discs reach no pole: only partial rings
-            g.nrings = g.irmax >= g.irmin ? g.irmax - g.irmin + 1 : 0;
-            double nacc = 0.0;
-            int T = 0;
-            for (int g0 = 0; g0 < g.nrings; g0 += 64) {
-                const int n = g.nrings - g0 < 64 ? g.nrings - g0 : 64;
-                // the first 64 rings go to the wave's cache row of member j
-                const size_t row = ((size_t)blockIdx.x * (size_t)K + (size_t)j) * 64;
-                const int NS = g0 == 0 ? ring_table(g, G, g.irmin, n, S, lane, -1, tab_f + row, tab_i + row)
-                                       : ring_table(g, G, g.irmin + g0, n, S, lane, -1);
-                T += lane < n ? S.ri[lane].y : 0;
-                nacc += walk_pv<0, ACC, false>(g.pv, S, NS, lane, g.K4, 0.0f, map);
-                wave_sync();
-            }
-            const double norm = wave_reduce_full<DppAdd>(nacc, 0.0);
-            T = wave_reduce_full<DppAdd>(T, 0);
-            if (T < 2) {
-                if (lane == j) acc_point<ACC>(map, my_p0, my_flux);
-                n_fb++;
+        // footprints of fewer than 2 pixels: whole flux to the particle's
+        // pixel (src/genmap.cpp:125-128)
+        if (lane < nm) {
+            if (my_T < 2) {
+                acc_point<ACC>(map, my_p0, my_flux);
             } else {
-                n_ext++;
-                n_contrib += (unsigned long long)T;
-                if (lane == j) {
-                    my_scale = (float)(my_flux / norm);
-                    my_on = true;
-                }
+                my_scale = (float)(my_flux / my_norm);
+                my_on = true;
             }
+        }
+        {
+            const uint64_t fb = __ballot(lane < nm && my_T < 2);
+            const uint64_t ex = __ballot(my_on);
+            n_fb += (unsigned long long)__popcll(fb);
+            n_ext += (unsigned long long)__popcll(ex);
+            n_contrib += (unsigned long long)wave_reduce_full<DppAdd>(my_on ? my_T : 0, 0);
         }
         // the cached ring terms are read back below
         __builtin_amdgcn_s_waitcnt(0);
@@ -555,7 +568,6 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
             const int b1 = b0 + BH - 1;
             // members meeting the band, in order; the cached ring terms of
             // the next one are loaded while the current one is walked
-            const size_t wrow = (size_t)blockIdx.x * (size_t)K;
             auto next_member = [&](int j) {
                 for (++j; j < nm; ++j) {
                     const int lo = wave_read(my_lo, j), hi = wave_read(my_hi, j);
@@ -566,10 +578,10 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
             auto fetch = [&](int j, float4 &f, int4 &i) {
                 const int lo = wave_read(my_lo, j), hi = wave_read(my_hi, j);
                 const int rlo = b0 > lo ? b0 : lo, rhi = b1 < hi ? b1 : hi;
-                const int idx = rlo - lo + lane;
-                if (lane <= rhi - rlo && idx < 64) {
-                    f = tab_f[(wrow + (size_t)j) * 64 + (size_t)idx];
-                    i = tab_i[(wrow + (size_t)j) * 64 + (size_t)idx];
+                const int row = wave_read(rstart, j) + (rlo - lo) + lane;
+                if (lane <= rhi - rlo && row < cache_rows) {
+                    f = tab_f[cache0 + (size_t)row];
+                    i = tab_i[cache0 + (size_t)row];
                 }
             };
             int j = next_member(-1);
@@ -587,10 +599,10 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
                 RingLite rl;
                 rl.cnt = 0;
                 if (on) {
-                    if (rlo - lo + lane < 64) {
+                    if (wave_read(rstart, j) + (rlo - lo) + lane < cache_rows) {
                         rl = ring_unpack(cf, ci);
                     } else {
-                        // rings past the cached 64 of a tall member
+                        // rows past the wave's cache
                         PGeo g;
                         g.phi = wave_read(my_phi, j);
                         g.z0 = wave_read(my_z0, j);
