@@ -2,10 +2,20 @@
 // generator (src/genmap_cuda_render.cpp:25-200), producing the CPU
 // generator's map (src/genmap.cpp:22-175).
 //
-// Each DataReader chunk is handed to the device as raw 48-byte records; the
-// per-particle pre-pass that the reference runs serially on the host
-// (src/genmap_cuda_render.cpp:102-167) runs on the device.  The map stays on
-// the device across chunks and comes back once, already divided by dOmega.
+// This file compiles against the reference's own headers (src/parameters.h,
+// src/datareader.h, src/mapgenerator.h) as well as this package's mirrors
+// of them: it uses only their public members, and the map constants come
+// from dmann_params_of().  Link it with libdmann.so in place of
+// genmap_cuda_render.o and chealpixrender.o.
+//
+// Each DataReader chunk is handed to the library as raw 48-byte records;
+// the library gathers chunks in pinned staging buffers and renders them in
+// large launches, the copy of one buffer overlapping the render of the
+// previous one, so small mem_buffer chunks (2048 in src/VL2_info.ini:57)
+// cost no more than large ones.  The per-particle pre-pass that the
+// reference runs serially on the host (src/genmap_cuda_render.cpp:102-167)
+// runs on the device.  The map stays on the device across chunks and comes
+// back once, already divided by dOmega.
 //
 // Accumulation form: DMANN_PRECISION=fp32 | compensated | fp64 in the
 // environment (default compensated, which meets the 1e-5 per-pixel bound
@@ -14,6 +24,9 @@
 // ann_core_nosom (cores from ./halocore/VL2_Halos.data or DMANN_HALO_FILE) -- the
 // reference links one src/flux/flux_<kind>.cpp per executable
 // (annmap_cuda_nosom, annmap_cuda_pwave, ..., src/Makefile:130-160).
+#include <rpc/types.h>
+#include <rpc/xdr.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,6 +35,7 @@
 #include <vector>
 
 #include "dmann.h"
+#include "dmann_params_of.h"
 #include "mapgenerator.h"
 
 static int precision_from_env()
@@ -49,9 +63,9 @@ void MapGenerator::start()
 {
     const int Npix = par_->map.Npix;
     dmann_params p;
-    par_->toDmannParams(&p, precision_from_env());
+    dmann_params_of(*par_, precision_from_env(), &p);
     p.flux_kind = flux_from_env();
-    int64_t cap = par_->memParts > 0 ? par_->memParts : 1;
+    const int64_t cap = par_->memParts > 0 ? par_->memParts : 1;
     dmann_ctx *ctx = nullptr;
     if (dmann_create(&ctx, &p, cap, 0) != DMANN_OK) {
         printf("HIP initialize error! %s\n", dmann_last_error(nullptr));

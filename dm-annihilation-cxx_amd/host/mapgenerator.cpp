@@ -1,4 +1,10 @@
 // MapGenerator bookkeeping and FITS output (src/mapgenerator.cpp:18-74).
+// Uses only the public Parameters members and the library's C ABI, so it
+// builds against the reference's headers too (the reference's own
+// mapgenerator.cpp needs HEALPix C++ for its FITS writer).
+#include <rpc/types.h>
+#include <rpc/xdr.h>
+
 #include "mapgenerator.h"
 
 #include <cstdio>
@@ -6,7 +12,7 @@
 #include <fstream>
 #include <iostream>
 
-#include "fitswriter.h"
+#include "dmann.h"
 
 MapGenerator::MapGenerator(Parameters *par, DataReader *reader)
     : par_(par), reader_(reader), isFinished_(false), map_(nullptr)
@@ -39,9 +45,8 @@ void MapGenerator::saveToFits(std::string, bool isVerbose)
     } else {
         std::cout << "Creating fits!..." << std::endl;
     }
-    std::string err;
-    if (!write_healpix_fits(fits, map_, (long long)par_->map.Npix, par_->map.Nside, &err))
-        fprintf(stderr, "%s\n", err.c_str());
+    if (dmann_write_healpix_fits(fits.c_str(), map_, (int64_t)par_->map.Npix, par_->map.Nside) != DMANN_OK)
+        fprintf(stderr, "cannot write %s\n", fits.c_str());
 }
 
 bool MapGenerator::isFinished() { return isFinished_; }

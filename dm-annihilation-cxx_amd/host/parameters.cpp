@@ -3,6 +3,7 @@
 // :183-289 (rotation) operation by operation, so the constants are
 // bit-identical to the reference build (checked against it in the tests).
 #include "parameters.h"
+#include "dmann_params_of.h"
 
 #include <cfloat>
 #include <cmath>
@@ -251,41 +252,6 @@ void Parameters::setupRotation()
         }
 }
 
-double Parameters::fluxFactor() const
-{
-    return pow(pow((natconst.c_in_cgs), 2) / (units.eV_in_cgs * 1.0e9), 2) / (units.Mpc_in_cgs * 1.0e-3) *
-           codeunits.annihilation_flux_to_cgs;
-}
-
-double Parameters::massCut() const
-{
-    float hires = FLT_MAX;
-    for (int i = 0; i < 10; i++)
-        if (params.particle_masses[i] > 0 && params.particle_masses[i] < hires) hires = (float)params.particle_masses[i];
-    return (double)hires * 1.1;
-}
-
-void Parameters::toDmannParams(dmann_params *p, int precision) const
-{
-    for (int i = 0; i < 3; i++) p->opos[i] = params.opos[i];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) p->rotmatrix[3 * i + j] = rotmatrix[i][j];
-    p->flux_factor = fluxFactor();
-    p->mass_cut = massCut();
-    p->nside = map.Nside;
-    p->precision = precision;
-    p->flux_kind = DMANN_FLUX_ANN_NOSOM;
-    p->reserved = 0;
-    // unit factor of src/flux/flux_decay.cpp:11-14
-    const double decay_flux_to_cgs = (codeunits.mass_to_cgs / pow(codeunits.length_to_cgs, 2));
-    p->decay_factor =
-        pow((natconst.c_in_cgs), 2) / (units.eV_in_cgs * 1.0e9) / (units.Mpc_in_cgs * 1.0e-3) * decay_flux_to_cgs;
-    p->vel_to_cgs = codeunits.velocity_to_cgs;
-    p->c_in_cgs = natconst.c_in_cgs;
-    for (int i = 0; i < 3; i++) p->cpos[i] = params.cpos[i];
-    p->footprint_pad = 0.0;
-}
-
 static std::string join(const double *d, int n)
 {
     std::string s;
@@ -333,7 +299,7 @@ extern "C" int dmann_params_from_ini(const char *ini, int precision, dmann_param
     if (!ini || !out) return DMANN_ERR_ARG;
     Parameters par;
     if (!Parameters::load(ini, par)) return DMANN_ERR_ARG;
-    par.toDmannParams(out, precision);
+    dmann_params_of(par, precision, out);
     if (mem_buffer) *mem_buffer = par.memParts;
     if (test_num) *test_num = par.testNum;
     return DMANN_OK;

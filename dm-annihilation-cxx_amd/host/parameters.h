@@ -39,13 +39,6 @@ public:
     bool isNative;
     bool isMask;
 
-    // flux factor of the nosom plug-in (src/flux/flux_ann_nosom.cpp:13-16)
-    double fluxFactor() const;
-    // (double)(float)min positive particle mass * 1.1 (src/genmap.cpp:52-58,82)
-    double massCut() const;
-    // fills the C-ABI constants of this map
-    void toDmannParams(dmann_params *p, int precision) const;
-
 private:
     double align_vector[3];
     double obs_pos[3];

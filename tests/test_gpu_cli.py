@@ -1,23 +1,35 @@
-"""End-to-end drop-in: annmap_hip -info <ini> reads a native particle file
-(with a mask) in mem_buffer chunks, renders on the GPU and writes the HEALPix
-FITS map; the map must match the oracle run on the same file (compensated
-form: 1e-5 per pixel, 1e-6 total), the FITS must round-trip, and a missing
-ini must exit non-zero with the reference's message."""
+"""End-to-end drop-in: <exe> -info <ini> reads a native particle file (with a
+mask) in mem_buffer chunks, renders on the GPU and writes the HEALPix FITS
+map; the map must match the oracle run on the same file (compensated form:
+1e-5 per pixel, 1e-6 total), the FITS must round-trip, and a missing ini
+must exit non-zero with the reference's message.
+
+Two executables: annmap_hip (the package's own driver and Parameters /
+DataReader mirrors) and oracle/_ref/annmap_ref -- the reference's own
+src/main.cpp, parameters.cpp, datareader.cpp linked with the package's
+MapGenerator::start (host/genmap_hip.cpp) compiled against the reference's
+headers, i.e. the link-time drop-in itself (built where the reference tree
+exists; skipped otherwise)."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import PKG, assert_map, rel_err_report
+from conftest import PKG, ROOT, assert_map, rel_err_report
 
 pytestmark = pytest.mark.gpu
 EXE = os.path.join(PKG, "annmap_hip")
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "annmap_ref")
 
 
-def test_annmap_hip_end_to_end(pw, orc, gpu, tmp_path):
-    print("case annmap_hip cli", flush=True)
-    assert os.path.exists(EXE), "build annmap_hip first (make -C dm-annihilation-cxx_amd)"
+@pytest.mark.parametrize("exe", [EXE, REF_EXE], ids=["annmap_hip", "reference_main"])
+def test_annmap_hip_end_to_end(pw, orc, gpu, tmp_path, exe):
+    print(f"case cli {os.path.basename(exe)}", flush=True)
+    if exe == REF_EXE and not os.path.exists(exe):
+        pytest.skip("the reference-linked drop-in is built only where the reference tree exists")
+    assert os.path.exists(exe), "build annmap_hip first (make -C dm-annihilation-cxx_amd)"
+    EXE_ = exe
     recs = pw.synth.to_records(pw.synth.make("nfw_1e5", n=6000))
     data = str(tmp_path / "parts.bin")
     mask = str(tmp_path / "mask.bin")
@@ -31,7 +43,7 @@ def test_annmap_hip_end_to_end(pw, orc, gpu, tmp_path):
     out = str(tmp_path / "sky.fits")
     ini = pw.write_ini(str(tmp_path / "run.ini"), nside=64, mem_buffer=1000, datafile=data, outputmap=out,
                        extra={"mask": mask})
-    r = subprocess.run([EXE, "-info", ini], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE_, "-info", ini], capture_output=True, text=True, timeout=120)
     print(r.stdout[-400:], r.stderr[-400:])
     assert r.returncode == 0
     assert "Total Conts" in r.stdout
@@ -45,7 +57,7 @@ def test_annmap_hip_end_to_end(pw, orc, gpu, tmp_path):
     print(rep)
     assert_map(rep, 1e-5, 1e-6)
     # a second run overwrites the existing file
-    r = subprocess.run([EXE, "-info", ini], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE_, "-info", ini], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "File exists" in r.stdout
 
 
