@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--precision", default="compensated", choices=["compensated", "fp32", "fp64"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-observer batch leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (backward) leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the FP64 GPU map that the per-pixel errors are measured against")
@@ -199,7 +201,9 @@ def cpu_baseline(pw, args, nside, n_full, c_full):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.build()
-    cores = min(16, os.cpu_count() or 1)
+    # the job's share of the host cores (OMP_NUM_THREADS where the machine
+    # sets it, else every core); nproc is reported beside it
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     hm = pw.synth.model(args.workload, "cpu")
     ini = os.path.join("/tmp", f"dmann_bench_{os.getpid()}.ini")
     pw.write_ini(ini, nside=nside)
@@ -227,7 +231,21 @@ def cpu_baseline(pw, args, nside, n_full, c_full):
 
     m, dt, c, rate = sized(cores, args.cpu_seconds)
     m1, dt1, c1, rate1 = sized(1, args.cpu_seconds / 3)
+    # the CPU-runnable case in full: 1e5 particles of the NFW host, nside 64
+    small = pw.synth.to_records(pw.synth.make("nfw_1e5"))
+    ini = os.path.join("/tmp", f"dmann_bench_{os.getpid()}_64.ini")
+    pw.write_ini(ini, nside=64)
+    cfg64 = oracle.config_from_params(oracle.params_from_ini(ini))
+    os.unlink(ini)
+    full = {}
+    for threads in (1, cores):
+        t0 = time.perf_counter()
+        _, st64 = oracle.genmap(small, cfg64, nthreads=threads)
+        sec = time.perf_counter() - t0
+        full[f"threads_{threads}"] = {"seconds": sec, "value": len(small) / sec / 1e6}
+    full["workload"] = f"nfw_1e5: {len(small)} particles (NFW host), nside 64, {st64['contributions']} footprint pixels"
     return {"value": n_full / (c_full / rate) / 1e6, "unit": "Mparticles/s", "cores": cores, "kind": "port",
+            "nproc": os.cpu_count(),
             "sample": f"{m} particles of the same model (16 index windows, near-observer share pro rata), "
                       f"nside {nside}: "
                       f"{c} footprint pixels in {dt:.1f} s on {cores} OpenMP threads, marginal rate over a "
@@ -236,7 +254,96 @@ def cpu_baseline(pw, args, nside, n_full, c_full):
             "sample_value": m / dt / 1e6,
             "single_core": {"value": n_full / (c_full / rate1) / 1e6,
                             "sample": f"{m1} particles, {c1} footprint pixels in {dt1:.1f} s on 1 thread, "
-                                      f"marginal rate {rate1:.3e} pixels/s"}}
+                                      f"marginal rate {rate1:.3e} pixels/s"},
+            "full_small_case": full}
+
+
+def batch_leg(pw, torch):
+    """64 observers on the 8 kpc circle in the z = 0 plane, 1e7 particles of
+    the NFW + 100 subhalo set, nside of each map drawn (seed 2) from
+    {256, 512, 1024}, compensated: one batch replayed from a hipGraph against
+    64 single-observer renders of the same maps."""
+    import numpy as np
+    nmaps = 64
+    rng = np.random.default_rng(2)
+    nsides = [int(x) for x in rng.choice([256, 512, 1024], nmaps)]
+    parts = pw.synth.make("nfw_sub100_1e7", "cuda")
+    plist = []
+    for k in range(nmaps):
+        a = 2 * math.pi * k / nmaps
+        plist.append(pw.params_for_observer(nsides[k], "compensated", obs_pos=(8 * math.cos(a), 8 * math.sin(a), 0.0)))
+    with pw.Batch(plist) as b:
+        b.render(parts, use_graph=True)
+        b.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.render(parts, use_graph=True)
+        b.synchronize()
+        t_graph = time.perf_counter() - t0
+        contrib = sum(b.stats(i)["contributions"] for i in range(nmaps))
+    t_single = 0.0
+    for k in range(nmaps):
+        with pw.Projector(plist[k], parts.shape[0]) as pr:
+            out = torch.empty(pr.npix, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pr.render(parts)
+            pr.get_map_device(out)
+            torch.cuda.synchronize()
+            t_single += time.perf_counter() - t0
+    n = parts.shape[0]
+    return {"workload": f"nfw_sub100_1e7: {n} particles, {nmaps} observers on the 8 kpc circle, nside from "
+                        "{256, 512, 1024} (seed 2), compensated",
+            "graph_ms": 1e3 * t_graph, "single_renders_ms": 1e3 * t_single,
+            "value": nmaps * n / t_graph / 1e6, "unit": "Mparticle-maps/s", "contributions": contrib}
+
+
+# Algorithmic FP64 work the adjoint does per footprint pixel, counted from
+# its formulas (k_adjoint; per-particle constants such as 1/r^2 hoisted):
+# pixel longitude and vector, cos of the angle (dot product) and sin (norm of
+# the cross product), the weight and its r- and direction-derivatives, and
+# the ten running sums -- 54 FP64 operations, plus sin, cos, sqrt, atan2,
+# exp and one division counted as 10 operations each: 114 per pixel.
+ADJOINT_FLOP_PER_PIXEL = 114
+FP64_PEAK_TFLOPS = 78.6
+
+
+def adjoint_leg(pw, torch):
+    """Backward pass of the adjoint workload: 1e6 particles of the NFW + 10
+    subhalo set (seed 3), nside 256, loss = sum (map - target)^2 / target."""
+    hm = pw.synth.model("nfw_sub10_1e6", "cuda")
+    parts = hm.generate()
+    target_parts = parts.clone()
+    target_parts[:, 0] *= 1.1
+    p = pw.params_for_observer(256, "fp64")
+    with pw.Projector(p, parts.shape[0]) as pr:
+        target = pw.project_map(target_parts, pr).detach().clone()
+        x = parts.clone().requires_grad_(True)
+        m = pw.project_map(x, pr)
+        mask = target > 0
+        loss = torch.where(mask, (m - target) ** 2 / torch.where(mask, target, torch.ones_like(target)),
+                           torch.zeros_like(m)).sum()
+        loss.backward()  # warm
+        g = torch.where(mask, 2 * (m - target) / torch.where(mask, target, torch.ones_like(target)),
+                        torch.zeros_like(m)).detach()
+        out = torch.empty(parts.shape[0], 6, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pr.adjoint(parts, g, out)
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pr.reset()
+        pr.render(parts)
+        pr.get_map_device(torch.empty(pr.npix, dtype=torch.float64, device="cuda"))
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        C = pr.stats()["contributions"]
+    achieved = ADJOINT_FLOP_PER_PIXEL * C / t_bwd / 1e12
+    return {"workload": f"nfw_sub10_1e6: {parts.shape[0]} particles, nside 256, FP64",
+            "backward_ms": 1e3 * t_bwd, "forward_fp64_ms": 1e3 * t_fwd, "contributions": C,
+            "roofline": {"bound": "fp64 valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP64_PEAK_TFLOPS, "flop_per_pixel": ADJOINT_FLOP_PER_PIXEL}}
 
 
 def main():
@@ -267,10 +374,12 @@ def main():
     plain = None
     pl_map = None
     if not args.no_plain and args.precision != "fp32":
-        pl = time_form(pw, torch, parts, nside, "fp32", args.steps, args.warmup)
+        # a side figure: a few steps suffice
+        pl_steps, pl_warm = min(args.steps, 3), min(args.warmup, 1)
+        pl = time_form(pw, torch, parts, nside, "fp32", pl_steps, pl_warm)
         pl_err = total_flux_check(torch, parts, params, pl["map"], nside)
         log(f"fp32: {pl['sec'] * 1e3:.1f} ms/step, kernels {pl['kernel_ms']}")
-        plain = {"value": n / pl["sec"] / 1e6, "ms_per_step": pl["sec"] * 1e3,
+        plain = {"value": n / pl["sec"] / 1e6, "ms_per_step": pl["sec"] * 1e3, "steps": pl_steps,
                  "kernel_ms": pl["kernel_ms"], "total_flux_rel_err": pl_err,
                  "compensated_over_plain_time": res["sec"] / pl["sec"]}
         pl_map = pl["map"]
@@ -342,9 +451,17 @@ def main():
     }
     if host_in is not None:
         line["host_input"] = host_in
-    if not args.no_cpu:
-        del parts
+    del parts
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        line["batch"] = batch_leg(pw, torch)
+        log(f"batch: {line['batch']}")
         torch.cuda.empty_cache()
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_leg(pw, torch)
+        log(f"adjoint: {line['adjoint']}")
+        torch.cuda.empty_cache()
+    if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pw, args, nside, n, C)
     print(json.dumps(line), flush=True)
 

@@ -17,6 +17,38 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a ROCm GPU (MI355X)")
 
 
+# Figures that tests measure at full size (error against the reference,
+# times), printed as one block at the end of the session so they appear
+# with the pass/fail summary even under -q.
+MEASURED = []
+
+
+def record(name: str, **figures):
+    MEASURED.append((name, figures))
+
+
+def progress(msg: str):
+    """A step of a long test finished: print it and append it to
+    gpurun_out/test_progress.log, so a run whose output is captured still
+    shows that it advances."""
+    print(msg, flush=True)
+    d = os.path.join(ROOT, "gpurun_out")
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "test_progress.log"), "a") as fh:
+        fh.write(msg + "\n")
+
+
+def pytest_terminal_summary(terminalreporter, exitstatus, config):
+    if not MEASURED:
+        return
+    terminalreporter.write_line("measured:")
+    for name, figs in MEASURED:
+        parts = []
+        for k, v in figs.items():
+            parts.append(f"{k}={v:.3g}" if isinstance(v, float) else f"{k}={v}")
+        terminalreporter.write_line(f"  {name}: " + " ".join(parts))
+
+
 @pytest.fixture(scope="session")
 def orc():
     import oracle

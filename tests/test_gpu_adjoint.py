@@ -12,13 +12,21 @@ central differences at 1e-4 are off by ~1e-3 relative for particles whose
 position gradient nearly cancels), evaluated in FP64 with the particle's
 footprint held fixed (the map is discontinuous where pixel centres cross
 the footprint edge; the adjoint differentiates the smooth part).
-Required: relative error <= 1e-4 per quantity (positions as a 3-vector).  The FP64 map path itself is pinned against the CPU oracle by
+Required: relative error <= 1e-4 per quantity (positions as a 3-vector).
+
+Exact reference: for the same 1000 particles, torch.autograd through an FP64
+torch restatement of the particle's contribution on the oracle's footprint
+(pixel centres from the oracle's pix2vec), contracted with the same dL/dmap:
+the adjoint kernel must agree to 1e-9 relative per quantity (positions as a
+3-vector).  The FP64 map path itself is pinned against the CPU oracle by
 test_gpu_parity.py.
 """
 import math
 
 import numpy as np
 import pytest
+
+from conftest import record
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +47,27 @@ def _contrib(q, pix_u, cfg_R, opos, K, dOmega):
     psi = np.arctan2(np.linalg.norm(np.cross(pix_u, v), axis=1), pix_u @ v)
     w = np.exp(-C_W * psi ** 2 / r ** 2)
     return flux * w / w.sum() / dOmega
+
+
+def _autograd_reference(q, pix_u, R, opos, K, dOmega, gpix):
+    """d/d(m, rho, h, x, y, z) of sum_p gpix_p c_p(q), c as in _contrib, by
+    torch.autograd in FP64."""
+    import torch
+    qt = torch.tensor(q[:6], dtype=torch.float64, requires_grad=True)
+    u = torch.from_numpy(np.ascontiguousarray(pix_u, dtype=np.float64))
+    m, rho, h = qt[0], qt[1], qt[2]
+    dv = qt[3:6] - torch.from_numpy(np.asarray(opos, dtype=np.float64))
+    d = torch.sqrt((dv * dv).sum())
+    flux = K * rho * m / (4.0 * math.pi * d * d)
+    t = torch.from_numpy(np.ascontiguousarray(R.T)) @ dv
+    v = torch.stack([-t[0], -t[1], t[2]]) / d
+    r = h / d
+    cr = torch.linalg.cross(u, v.expand_as(u), dim=1)
+    psi = torch.atan2(cr.norm(dim=1), u @ v)
+    w = torch.exp(-C_W * psi ** 2 / r ** 2)
+    c = flux * w / w.sum() / dOmega
+    (c * torch.from_numpy(np.ascontiguousarray(gpix, dtype=np.float64))).sum().backward()
+    return qt.grad.numpy()
 
 
 def test_adjoint_vs_finite_differences(pw, orc, gpu, tmp_path):
@@ -76,6 +105,8 @@ def test_adjoint_vs_finite_differences(pw, orc, gpu, tmp_path):
     recs = pw.synth.to_records(parts)
     pick = np.random.default_rng(6).choice(len(parts), 1000, replace=False)
     worst = {"m": 0.0, "rho": 0.0, "h": 0.0, "pos": 0.0}
+    exact = {"m": 0.0, "rho": 0.0, "h": 0.0, "pos": 0.0}
+    Gfull = np.where(T > 0, 2 * (M - T) / np.where(T > 0, T, 1.0), 0.0)
     crossed = 0
     for i in pick:
         q = parts[i].numpy().astype(np.float64)
@@ -104,6 +135,11 @@ def test_adjoint_vs_finite_differences(pw, orc, gpu, tmp_path):
         fr = fd(1, 1e-4 * q[1])
         fh = fd(2, 1e-4 * q[2])
         fp = np.array([fd(3 + a, 1e-4 * d) for a in range(3)])
+        ga = _autograd_reference(q, u, R, opos, cfg.flux_factor, dOmega, Gfull[pix])
+        exact["m"] = max(exact["m"], abs(g6[i][0] - ga[0]) / abs(ga[0]))
+        exact["rho"] = max(exact["rho"], abs(g6[i][1] - ga[1]) / abs(ga[1]))
+        exact["h"] = max(exact["h"], abs(g6[i][2] - ga[2]) / max(abs(ga[2]), 1e-300))
+        exact["pos"] = max(exact["pos"], float(np.linalg.norm(g6[i][3:6] - ga[3:6]) / np.linalg.norm(ga[3:6])))
         g = grads[i]
         assert np.allclose(g6[i].astype(np.float32), grads[i][:6], rtol=1e-6, atol=0)
         assert not grads[i][6:].any()
@@ -118,5 +154,10 @@ def test_adjoint_vs_finite_differences(pw, orc, gpu, tmp_path):
             crossed += 1
         del g
     print("worst relative errors vs FD:", worst, "footprints changed by the h step:", crossed)
+    print("worst relative errors vs the FP64 autograd reference:", exact)
+    record("adjoint 1e6 nside256 vs FP64 autograd", **{f"max_rel_{k}": v for k, v in exact.items()})
+    record("adjoint 1e6 nside256 vs finite differences", **{f"max_rel_{k}": v for k, v in worst.items()})
+    for k, v in exact.items():
+        assert v <= 1e-9, (k, v)
     for k, v in worst.items():
         assert v <= 1e-4, (k, v)

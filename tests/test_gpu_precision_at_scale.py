@@ -1,0 +1,103 @@
+"""Precision at scale: the 1.1e9-particle model (NFW host + 2000 subhalos +
+1e4 particles within 2 kpc of the observer, seed 4) into an nside-4096 map.
+
+The compensated and plain FP32 forms are compared over all 201M pixels
+with the FP64 form -- FP64 weights computed as the reference computes them
+and FP64 accumulation, pinned against the CPU oracle by test_gpu_parity.py
+-- rendered in 8 accumulating chunks.  Required: compensated <= 1e-5 per
+pixel above 1e-6 of the maximum (the same bar in absolute terms below it,
+empty pixels exactly empty), <= 1e-6 on the total, and the total-flux
+invariant of src/genmap.cpp:161-172 (sum(map) dOmega = sum of the kept
+particles' fluxes, FP64 from the same inputs) to 1e-6; compensated may take
+at most 1.3x the plain FP32 time.  Plain FP32's errors are recorded (at
+this size they exceed the compensated bar, which is why the compensated
+form exists).
+"""
+import math
+import time
+
+import pytest
+
+from conftest import progress, record
+
+pytestmark = pytest.mark.gpu
+
+NSIDE = 4096
+
+
+def _pixel_errors(m, ref):
+    big = ref > 1e-6 * ref.max()
+    rel = ((m - ref).abs() / ref.abs().clamp_min(1e-300))[big]
+    below = float((m - ref)[~big].abs().max() / ref.max()) if bool((~big).any()) else 0.0
+    zero_mismatch = int(((ref == 0) & (m != 0)).sum())
+    return {"max_rel": float(rel.max()), "pixels_checked": int(big.sum()), "below_floor_abs": below,
+            "zero_mismatch": zero_mismatch, "total_rel": float(abs(m.sum() - ref.sum()) / ref.sum())}
+
+
+def _total_flux(torch, parts, p):
+    opos = torch.tensor(list(p.opos), dtype=torch.float64, device=parts.device)
+    tot = torch.zeros((), dtype=torch.float64, device=parts.device)
+    for i in range(0, parts.shape[0], 1 << 24):
+        c = parts[i:i + (1 << 24)]
+        keep = (c[:, 0].double() < p.mass_cut) & (c[:, 1] >= 0)
+        d2 = ((c[:, 3:6].double() - opos) ** 2).sum(1)
+        f = p.flux_factor * c[:, 1].double() * c[:, 0].double() / (4.0 * math.pi * d2)
+        tot += torch.where(keep, f, torch.zeros_like(f)).sum()
+    return float(tot)
+
+
+def _render(pw, torch, parts, precision, chunks=1):
+    p = pw.params_for_observer(NSIDE, precision)
+    n = parts.shape[0]
+    cap = (n + chunks - 1) // chunks
+    out = torch.empty(12 * NSIDE * NSIDE, dtype=torch.float64, device=parts.device)
+    with pw.Projector(p, cap) as pr:
+        pr.render(parts[: 1 << 20])  # warm-up (module load), then cleared
+        pr.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(0, n, cap):
+            pr.render(parts[i:i + cap])
+            if chunks > 1:
+                pr.synchronize()
+                progress(f"precision at scale, {precision}: {min(n, i + cap)} of {n} particles rendered")
+        pr.get_map_device(out)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        st = pr.stats()
+    return out, sec, st, p
+
+
+@pytest.mark.timeout(900)
+def test_headline_model_nside4096(pw, gpu):
+    import torch
+    print("case precision at scale: 1.1e9 particles, nside 4096", flush=True)
+    hm = pw.synth.model("nfw_sub2000_near_1p1e9", "cuda")
+    parts = hm.generate()
+    torch.cuda.synchronize()
+    progress("precision at scale: 1.1e9 particles generated")
+    comp, t_comp, st, p = _render(pw, torch, parts, "compensated")
+    progress(f"precision at scale: compensated {t_comp:.1f} s")
+    plain, t_plain, st_plain, _ = _render(pw, torch, parts, "fp32")
+    progress(f"precision at scale: plain {t_plain:.1f} s")
+    ref, t_ref, st_ref, _ = _render(pw, torch, parts, "fp64", chunks=8)
+    progress(f"precision at scale: fp64 {t_ref:.1f} s")
+    for key in ("kept", "subpixel", "fallback", "extended", "contributions", "large"):
+        assert st[key] == st_ref[key] == st_plain[key], key
+    e_comp = _pixel_errors(comp, ref)
+    e_plain = _pixel_errors(plain, ref)
+    total = _total_flux(torch, parts, p)
+    d_omega = 4.0 * math.pi / (12 * NSIDE * NSIDE)
+    inv_comp = abs(float(comp.sum()) * d_omega - total) / total
+    inv_ref = abs(float(ref.sum()) * d_omega - total) / total
+    record("1.1e9 nside4096 compensated vs FP64", max_rel=e_comp["max_rel"], total_rel=e_comp["total_rel"],
+           pixels=e_comp["pixels_checked"], flux_invariant=inv_comp, seconds=t_comp,
+           over_plain_time=t_comp / t_plain, contributions=st["contributions"])
+    record("1.1e9 nside4096 plain FP32 vs FP64", max_rel=e_plain["max_rel"], total_rel=e_plain["total_rel"],
+           seconds=t_plain)
+    record("1.1e9 nside4096 FP64", flux_invariant=inv_ref, seconds=t_ref)
+    print(e_comp, e_plain, inv_comp, inv_ref, flush=True)
+    assert e_comp["max_rel"] <= 1e-5 and e_comp["total_rel"] <= 1e-6
+    assert e_comp["below_floor_abs"] <= 1e-11 and e_comp["zero_mismatch"] == 0
+    assert inv_comp <= 1e-6 and inv_ref <= 1e-12
+    assert t_comp <= 1.3 * t_plain

@@ -57,8 +57,10 @@ def test_batch_reports_undefined_direction(pw, gpu, chunk):
     print(f"case batch domain error chunk={chunk}", flush=True)
     plist = [_observer_params(pw, k, 4, 64, "compensated") for k in range(4)]
     parts = pw.synth.make("nfw_1e5", n=2000)
-    # one particle exactly at observer 2's position (8 cos pi, 8 sin pi, 0)
-    o = plist[2].opos
+    # one particle exactly at observer 0's position (8, 0, 0), which FP32
+    # represents exactly (the other observers' positions are not)
+    o = plist[0].opos
+    assert list(o) == [8.0, 0.0, 0.0]
     parts[7, 3], parts[7, 4], parts[7, 5] = o[0], o[1], o[2]
     parts = parts.to(gpu)
     with pw.Batch(plist) as b:
@@ -67,9 +69,9 @@ def test_batch_reports_undefined_direction(pw, gpu, chunk):
         b.render(parts, use_graph=False)
         with pytest.raises(pw.DmannError) as ei:
             b.synchronize()
-        assert ei.value.code == DOMAIN and "map 2" in str(ei.value)
+        assert ei.value.code == DOMAIN and "map 0" in str(ei.value)
     # the single-map path says the same for that observer
-    with pw.Projector(plist[2], parts.shape[0]) as pr:
+    with pw.Projector(plist[0], parts.shape[0]) as pr:
         pr.render(parts)
         with pytest.raises(pw.DmannError) as ei:
             pr.synchronize()
