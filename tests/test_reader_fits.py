@@ -140,3 +140,59 @@ def test_product_reader_errors(pw, tipsy_kat, tmp_path):
     ini = pw.write_ini(str(tmp_path / "n.ini"), nside=16, extra={"datafile": str(tmp_path / "missing.bin")})
     with pytest.raises(pw.DmannError):
         pw.read_particles(ini)
+
+
+CFITSIO = "/opt/conda/lib/libcfitsio.so"
+
+
+@pytest.mark.skipif(not os.path.exists(CFITSIO), reason="cfitsio is not installed here")
+@pytest.mark.parametrize("nside", [2, 64])
+def test_fits_read_by_cfitsio(pw, tmp_path, nside):
+    """The FITS map read back by an independent reader, cfitsio (the
+    library the reference's HEALPix fitshandle writes with,
+    src/mapgenerator.cpp:30-62): second HDU a binary table with the
+    HEALPix keywords, one float64 column holding the map in RING order."""
+    import ctypes as C
+    L = C.CDLL(CFITSIO)
+    npix = 12 * nside * nside
+    m = np.random.default_rng(1).standard_normal(npix) * 1e-3
+    path = str(tmp_path / "map.fits")
+    pw.write_fits(path, m, nside)
+    f = C.c_void_p()
+    st = C.c_int(0)
+    assert L.ffopen(C.byref(f), path.encode(), 0, C.byref(st)) == 0, st.value
+    nhdu = C.c_int(0)
+    L.ffthdu(f, C.byref(nhdu), C.byref(st))
+    assert nhdu.value == 2
+    ext = C.c_int(0)
+    assert L.ffmahd(f, 2, C.byref(ext), C.byref(st)) == 0 and ext.value == 2  # BINARY_TBL
+
+    def key_long(name):
+        v = C.c_long(0)
+        assert L.ffgkyj(f, name.encode(), C.byref(v), None, C.byref(st)) == 0, (name, st.value)
+        return v.value
+
+    def key_str(name):
+        v = C.create_string_buffer(72)
+        assert L.ffgkys(f, name.encode(), v, None, C.byref(st)) == 0, (name, st.value)
+        return v.value.decode().strip()
+
+    assert key_long("NSIDE") == nside and key_long("LASTPIX") == npix - 1
+    assert key_str("ORDERING") == "RING" and key_str("PIXTYPE") == "HEALPIX"
+    ncol = C.c_int(0)
+    L.ffgncl(f, C.byref(ncol), C.byref(st))
+    assert ncol.value == 1
+    typecode, repeat, width = C.c_int(0), C.c_long(0), C.c_long(0)
+    assert L.ffgtcl(f, 1, C.byref(typecode), C.byref(repeat), C.byref(width), C.byref(st)) == 0
+    assert typecode.value == 82  # TDOUBLE
+    nrows = C.c_long(0)
+    L.ffgnrw(f, C.byref(nrows), C.byref(st))
+    assert nrows.value * repeat.value == npix
+    out = np.zeros(npix, dtype=np.float64)
+    anynul = C.c_int(0)
+    L.ffgcvd.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_longlong, C.c_double,
+                         C.c_void_p, C.c_void_p, C.c_void_p]
+    assert L.ffgcvd(f, 1, 1, 1, npix, 0.0, out.ctypes.data, C.byref(anynul), C.byref(st)) == 0, st.value
+    L.ffclos(f, C.byref(st))
+    assert st.value == 0
+    assert np.array_equal(out, m)
