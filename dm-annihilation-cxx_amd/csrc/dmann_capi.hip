@@ -31,8 +31,8 @@ struct dmann_ctx {
     int acc = kAccF32;
     DevParams P{};
     int64_t max_parts = 0;
+    int64_t rec_cap = 0; // records the render buffers hold (>= max_parts)
     int64_t npix = 0;
-    Particle *d_parts = nullptr; // staging for host particles
     DiscRec *d_recs = nullptr;
     unsigned *d_counts = nullptr;
     unsigned long long *d_stats = nullptr;
@@ -44,6 +44,16 @@ struct dmann_ctx {
     bool timing = false;
     std::vector<hipEvent_t> events; // 5 per timed render call
     hipEvent_t order_ev = nullptr;  // orders a new stream after the old one
+    // host input: dmann_render gathers particles in two pinned staging
+    // buffers and renders kStage of them per launch, the copy of one buffer
+    // (on copy_stream) overlapping the render of the other
+    dmann_particle *pin[2] = {nullptr, nullptr};
+    Particle *dstage[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {nullptr, nullptr};   // H2D of a buffer done
+    hipEvent_t consumed[2] = {nullptr, nullptr}; // render that read a buffer done
+    hipStream_t copy_stream = nullptr;
+    int cur = 0;      // buffer being filled
+    int64_t fill = 0; // particles in it
     // halo cores of the ANN_CORE_NOSOM plug-in (device copies)
     float4 *d_halos = nullptr;
     int *d_cell_start = nullptr;
@@ -100,7 +110,7 @@ static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
     c->gc.k = 0;
     c->gc.nside_c = nside_c;
     if (k <= 0 || c->acc == kAccF64) return hipSuccess;
-    const int64_t cap = c->max_parts ? c->max_parts : 1;
+    const int64_t cap = c->rec_cap ? c->rec_cap : 1;
     const int64_t nbins = 12LL * nside_c * nside_c * kSizeBuckets;
     const size_t rows = (size_t)c->gc.group_blocks * (size_t)k * 64;
     hipError_t e;
@@ -137,11 +147,28 @@ static void free_halos(dmann_ctx *c)
     c->halos_set = false;
 }
 
+static void free_stage(dmann_ctx *c)
+{
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    for (int b = 0; b < 2; b++) {
+        (void)hipHostFree(c->pin[b]);
+        (void)hipFree(c->dstage[b]);
+        if (c->copied[b]) (void)hipEventDestroy(c->copied[b]);
+        if (c->consumed[b]) (void)hipEventDestroy(c->consumed[b]);
+        c->pin[b] = nullptr;
+        c->dstage[b] = nullptr;
+        c->copied[b] = c->consumed[b] = nullptr;
+    }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    c->copy_stream = nullptr;
+    c->fill = 0;
+}
+
 static void free_all(dmann_ctx *c)
 {
+    free_stage(c);
     free_group(c);
     free_halos(c);
-    hipFree(c->d_parts);
     hipFree(c->d_recs);
     hipFree(c->d_counts);
     hipFree(c->d_stats);
@@ -149,7 +176,6 @@ static void free_all(dmann_ctx *c)
     hipFree(c->map.lo);
     hipFree(c->map.f64);
     hipFree(c->d_out);
-    c->d_parts = nullptr;
     c->d_recs = nullptr;
     c->d_counts = nullptr;
     c->d_stats = nullptr;
@@ -178,8 +204,73 @@ void fill_devparams(const dmann_params *p, DevParams &P, int large_pix)
     P.c_in_cgs = p->c_in_cgs;
     for (int i = 0; i < 3; i++) P.cpos[i] = p->cpos[i];
     P.footprint_pad = p->footprint_pad;
+    P.subpixel_sort = 1;
 }
 } // namespace dmann
+
+// Particles per host-input launch: small reader chunks (the reference's
+// mem_buffer, 2048 in src/VL2_info.ini:57) are gathered into launches of
+// this size, larger calls are split.
+constexpr int64_t kStage = 1 << 22;
+
+// Pinned and device staging buffers, and render buffers for kStage records.
+static int ensure_stage(dmann_ctx *c)
+{
+    if (c->pin[0]) return DMANN_OK;
+    if (c->rec_cap < kStage) {
+        const int k = c->gc.k, nside_c = c->gc.nside_c;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->d_recs);
+        c->d_recs = nullptr;
+        c->rec_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_recs, kStage * sizeof(DiscRec)));
+        c->rec_cap = kStage;
+        if (k > 0) HIPCHK(c, alloc_group(c, k, nside_c));
+    }
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(c, hipHostMalloc((void **)&c->pin[b], kStage * sizeof(dmann_particle), hipHostMallocDefault));
+        HIPCHK(c, hipMalloc(&c->dstage[b], kStage * sizeof(Particle)));
+        HIPCHK(c, hipEventCreateWithFlags(&c->copied[b], hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->consumed[b], hipEventDisableTiming));
+    }
+    HIPCHK(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    return DMANN_OK;
+}
+
+// Renders the particles gathered in the current staging buffer.
+static int stage_flush(dmann_ctx *c)
+{
+    if (c->fill == 0) return DMANN_OK;
+    const int b = c->cur;
+    // the copy may overwrite the device buffer once the render that last
+    // read it is done; the render waits for its copy
+    HIPCHK(c, hipStreamWaitEvent(c->copy_stream, c->consumed[b], 0));
+    HIPCHK(c, hipMemcpyAsync(c->dstage[b], c->pin[b], c->fill * sizeof(Particle), hipMemcpyHostToDevice,
+                             c->copy_stream));
+    HIPCHK(c, hipEventRecord(c->copied[b], c->copy_stream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->copied[b], 0));
+    hipEvent_t *ev = nullptr;
+    if (c->timing) {
+        for (int i = 0; i < 5; i++) {
+            hipEvent_t e;
+            HIPCHK(c, hipEventCreate(&e));
+            c->events.push_back(e);
+        }
+        ev = &c->events[c->events.size() - 5];
+    }
+    HIPCHK(c, launch_render(c->acc, c->dstage[b], c->fill, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg,
+                            c->stream, ev, &c->gc));
+    HIPCHK(c, hipEventRecord(c->consumed[b], c->stream));
+    c->cur ^= 1;
+    c->fill = 0;
+    return DMANN_OK;
+}
+
+#define FLUSH(c)                                                                                   \
+    do {                                                                                           \
+        const int f_ = stage_flush(c);                                                             \
+        if (f_ != DMANN_OK) return f_;                                                             \
+    } while (0)
 
 extern "C" {
 
@@ -208,6 +299,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     c->device = device;
     c->acc = p->precision == DMANN_PREC_FP32 ? kAccF32 : p->precision == DMANN_PREC_COMPENSATED ? kAccComp : kAccF64;
     c->max_parts = max_parts;
+    c->rec_cap = max_parts;
     c->npix = npix_of(p->nside);
     fill_devparams(p, c->P, 1 << 22);
 
@@ -222,6 +314,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     c->cfg.prepass_blocks = dev_cus * 64;
     c->cfg.medium_blocks = dev_cus * 8;
     c->cfg.large_blocks = dev_cus * 4;
+    c->cfg.sky_blocks = dev_cus * 8;
     auto bail = [&](const char *what, hipError_t err) {
         snprintf(g_create_msg, sizeof g_create_msg, "%s: %s", what, hipGetErrorString(err));
         free_all(c);
@@ -237,7 +330,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
         if (c->acc == kAccComp && (e = hipMalloc(&c->map.lo, np * sizeof(float))) != hipSuccess)
             return bail("hipMalloc map lo", e);
     }
-    if ((e = hipMalloc(&c->d_recs, (max_parts ? max_parts : 1) * sizeof(DiscRec))) != hipSuccess)
+    if ((e = hipMalloc(&c->d_recs, (c->rec_cap ? c->rec_cap : 1) * sizeof(DiscRec))) != hipSuccess)
         return bail("hipMalloc records", e);
     if ((e = hipMalloc(&c->d_counts, 4 * sizeof(unsigned))) != hipSuccess) return bail("hipMalloc counts", e);
     if ((e = hipMalloc(&c->d_stats, kNumStats * sizeof(unsigned long long))) != hipSuccess)
@@ -265,6 +358,23 @@ int dmann_set_tuning(dmann_ctx *c, int medium_blocks_per_cu, int large_blocks_pe
     return DMANN_OK;
 }
 
+int dmann_set_subpixel_sort(dmann_ctx *c, int on)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (on != 0 && on != 1) return fail(c, DMANN_ERR_ARG, "subpixel sort flag %d is not 0 or 1", on);
+    c->P.subpixel_sort = on;
+    return DMANN_OK;
+}
+
+int dmann_set_sky_gather(dmann_ctx *c, int blocks_per_cu)
+{
+    if (!c) return DMANN_ERR_ARG;
+    if (blocks_per_cu < 0 || blocks_per_cu > 32)
+        return fail(c, DMANN_ERR_ARG, "sky gather workgroups per CU %d outside 0..32", blocks_per_cu);
+    c->cfg.sky_blocks = (c->cus > 0 ? c->cus : 1) * blocks_per_cu;
+    return DMANN_OK;
+}
+
 int dmann_set_grouping(dmann_ctx *c, int k, int nside_c)
 {
     if (!c) return DMANN_ERR_ARG;
@@ -272,6 +382,7 @@ int dmann_set_grouping(dmann_ctx *c, int k, int nside_c)
     if (nside_c < 1 || nside_c > c->P.nside || nside_c > kMaxSortNside)
         return fail(c, DMANN_ERR_ARG, "sort cell nside %d outside 1..min(nside, %d)", nside_c, kMaxSortNside);
     HIPCHK(c, hipSetDevice(c->device));
+    FLUSH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, alloc_group(c, k, nside_c));
     return DMANN_OK;
@@ -283,6 +394,7 @@ int dmann_set_halo_cores(dmann_ctx *c, const float *halos, int64_t n)
     if (n < 0 || (n > 0 && !halos) || n > (int64_t)INT32_MAX / 8)
         return fail(c, DMANN_ERR_ARG, "bad halo table (%lld rows)", (long long)n);
     HIPCHK(c, hipSetDevice(c->device));
+    FLUSH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_halos(c);
     c->halos_set = true;
@@ -361,6 +473,7 @@ int dmann_set_stream(dmann_ctx *c, void *stream)
     if (!c) return DMANN_ERR_ARG;
     const hipStream_t s = (hipStream_t)stream;
     if (s == c->stream) return DMANN_OK;
+    FLUSH(c);
     // work queued on the old stream (map clears, renders, reads) comes
     // before anything enqueued on the new one
     HIPCHK(c, hipSetDevice(c->device));
@@ -383,6 +496,7 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
         return fail(c, DMANN_ERR_CAPACITY, "%lld particles exceed the context capacity %lld", (long long)n,
                     (long long)c->max_parts);
     HIPCHK(c, hipSetDevice(c->device));
+    FLUSH(c);
     hipEvent_t *ev = nullptr;
     if (c->timing) {
         for (int i = 0; i < 5; i++) {
@@ -418,6 +532,7 @@ int dmann_timing(dmann_ctx *c, int enable, double *ms, int *calls)
 {
     if (!c) return DMANN_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
+    FLUSH(c);
     if (ms || calls) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         double acc[4] = {0, 0, 0, 0};
@@ -449,12 +564,20 @@ int dmann_render(dmann_ctx *c, const dmann_particle *host_parts, int64_t n)
                     (long long)c->max_parts);
     HIPCHK(c, hipSetDevice(c->device));
     if (c->gc.k > 0 && !group_ready(c->gc)) return fail(c, DMANN_ERR_HIP, "grouping buffers are missing");
-    if (!c->d_parts) HIPCHK(c, hipMalloc(&c->d_parts, c->max_parts * sizeof(Particle)));
-    HIPCHK(c, hipMemcpyAsync(c->d_parts, host_parts, n * sizeof(Particle), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_render(c->acc, c->d_parts, n, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg,
-                            c->stream, nullptr, &c->gc));
-    // the staging buffer is reused by the next call: wait for this one
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int e = ensure_stage(c);
+    if (e != DMANN_OK) return e;
+    // the caller's buffer is copied into pinned staging before returning
+    const dmann_particle *src = host_parts;
+    int64_t left = n;
+    while (left > 0) {
+        if (c->fill == 0) HIPCHK(c, hipEventSynchronize(c->copied[c->cur]));
+        const int64_t take = left < kStage - c->fill ? left : kStage - c->fill;
+        memcpy(c->pin[c->cur] + c->fill, src, (size_t)take * sizeof(dmann_particle));
+        c->fill += take;
+        src += take;
+        left -= take;
+        if (c->fill == kStage) FLUSH(c);
+    }
     return DMANN_OK;
 }
 
@@ -462,6 +585,7 @@ int dmann_synchronize(dmann_ctx *c)
 {
     if (!c) return DMANN_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
+    FLUSH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     unsigned long long bad = 0;
     HIPCHK(c, hipMemcpy(&bad, c->d_stats + kStatBad, sizeof bad, hipMemcpyDeviceToHost));
@@ -476,6 +600,7 @@ int dmann_get_map_device(dmann_ctx *c, double *dev_map)
     if (!c) return DMANN_ERR_ARG;
     if (!dev_map) return fail(c, DMANN_ERR_ARG, "null map pointer");
     HIPCHK(c, hipSetDevice(c->device));
+    FLUSH(c);
     HIPCHK(c, launch_finalize(c->acc, c->map, c->npix, c->P.dOmega, dev_map, c->stream));
     return DMANN_OK;
 }
@@ -485,6 +610,7 @@ int dmann_get_map(dmann_ctx *c, double *host_map)
     if (!c) return DMANN_ERR_ARG;
     if (!host_map) return fail(c, DMANN_ERR_ARG, "null map pointer");
     HIPCHK(c, hipSetDevice(c->device));
+    FLUSH(c);
     if (!c->d_out) HIPCHK(c, hipMalloc(&c->d_out, c->npix * sizeof(double)));
     HIPCHK(c, launch_finalize(c->acc, c->map, c->npix, c->P.dOmega, c->d_out, c->stream));
     HIPCHK(c, hipMemcpyAsync(host_map, c->d_out, c->npix * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -495,6 +621,7 @@ int dmann_reset(dmann_ctx *c)
 {
     if (!c) return DMANN_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
+    c->fill = 0; // gathered particles not yet rendered are dropped with the map
     HIPCHK(c, launch_clear(c->acc, c->map, c->npix, c->d_stats, c->stream));
     return DMANN_OK;
 }
@@ -503,6 +630,7 @@ int dmann_get_stats(dmann_ctx *c, dmann_stats *st)
 {
     if (!c || !st) return DMANN_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
+    FLUSH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     unsigned long long h[kNumStats];
     HIPCHK(c, hipMemcpy(h, c->d_stats, sizeof h, hipMemcpyDeviceToHost));
@@ -558,6 +686,64 @@ int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)
             mism++;
     }
     return mism;
+}
+
+int64_t dmann_host_sky_check(const dmann_params *p, const dmann_particle *q)
+{
+    if (!p || !q || p->nside < 1 || p->nside > kMaxNside) return -1;
+    DevParams P;
+    fill_devparams(p, P);
+    DiscRec rec;
+    bool bad = false;
+    const int cls = prepass_one(P, *reinterpret_cast<const Particle *>(q), rec, bad);
+    if (bad || cls != kClsLarge) return -2;
+    const PGeo g = particle_geo(rec, P.nside, P.footprint_pad);
+    // the per-particle ring walk: weight of every footprint pixel
+    std::vector<float> wmap((size_t)P.npix, -1.0f);
+    int64_t mism = 0, k = 0;
+    for (int gi = 0; gi < g.nrings; ++gi) {
+        const RingRec rr = ring_setup(g, P.nside, gi);
+        int abase, kw;
+        ring_addr(rr, (int)k, abase, kw);
+        for (int t = 0; t < rr.cnt; ++t, ++k) {
+            const int px = ring_pixel(abase, kw, rr.nr, (int)k);
+            if (wmap[px] >= 0.0f) mism++; // a pixel twice
+            wmap[px] = ring_weight32(g, rr, t);
+        }
+    }
+    // the gather: rings of the range, segments, slots
+    int rlo, rhi;
+    sky_ring_range(g, P.nside, rlo, rhi);
+    int64_t seen = 0;
+    for (int r = rlo; r <= rhi; ++r) {
+        int startpix, nr, shifted;
+        ring_info(P.nside, r, startpix, nr, shifted);
+        const RingRec rr = ring_setup(g, P.nside, ring_index(g, r));
+        if (rr.startpix != startpix) {
+            mism++;
+            continue;
+        }
+        int c0 = rr.col0 % nr;
+        if (c0 < 0) c0 += nr;
+        for (int s0 = 0; s0 < nr; s0 += kSkySeg) {
+            const int ws = nr - s0 < kSkySeg ? nr - s0 : kSkySeg;
+            const SegSpan sp = seg_span(c0, rr.cnt, nr, s0, ws);
+            for (int i = 0; i < sp.na + sp.nb; ++i) {
+                const int t = i < sp.na ? sp.u + i : i - sp.na;
+                const int slot = seg_slot(c0, t, nr, s0);
+                if (t < 0 || t >= rr.cnt || slot < 0 || slot >= ws) {
+                    mism++;
+                    continue;
+                }
+                const int px = startpix + s0 + slot;
+                const float w = ring_weight32(g, rr, t);
+                if (wmap[px] != w) mism++; // not in the footprint, or another weight
+                wmap[px] = -2.0f;          // taken
+                seen++;
+            }
+        }
+    }
+    return mism + (k - seen);
 }
 
 int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int32_t *pix, float *w, int64_t cap,

@@ -132,6 +132,51 @@ __device__ __forceinline__ void acc_point(const MapRef &m, int addr, double flux
     }
 }
 
+// Sub-pixel deposits of one wavefront (src/genmap.cpp:106-110: the whole
+// flux to the RING pixel of the direction), reduced before they reach the
+// map: the lanes' (pixel, flux) pairs are sorted by pixel across the
+// wavefront (bitonic network on the unique keys pixel << 6 | lane, lanes
+// without a deposit last), each run of equal pixels is summed in FP64 by a
+// segmented scan (runs are contiguous once sorted), and the last lane of a
+// run deposits its sum -- one atomic per distinct pixel instead of one per
+// particle, which is what keeps many particles in one pixel (a distant
+// halo at low nside) from serialising on that pixel's address.  Every lane
+// of the wavefront must call it.
+template <int ACC>
+__device__ __forceinline__ void wave_subpixel_deposit(const MapRef &m, bool has, int pix, double flux)
+{
+    const int lane = __lane_id();
+    uint64_t key = has ? (((uint64_t)(unsigned)pix << 6) | (uint64_t)lane) : ~0ull;
+    double val = has ? flux : 0.0;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            // compare-exchange with lane ^ j: the lower lane of the pair
+            // keeps the smaller key in an ascending k-sequence, the larger
+            // one in a descending one
+            const uint64_t ok = (uint64_t)__shfl_xor((long long)key, j, 64);
+            const double ov = __shfl_xor(val, j, 64);
+            const bool lower = (lane & j) == 0;
+            const bool up = (lane & k) == 0;
+            if ((lower == up) ? (ok < key) : (ok > key)) {
+                key = ok;
+                val = ov;
+            }
+        }
+    }
+    const unsigned p = (unsigned)(key >> 6);
+    double run = val;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double up = __shfl_up(run, o, 64);
+        const unsigned upix = (unsigned)((uint64_t)__shfl_up((long long)key, o, 64) >> 6);
+        if (lane >= o && upix == p) run += up;
+    }
+    const unsigned next = (unsigned)((uint64_t)__shfl_down((long long)key, 1, 64) >> 6);
+    if (key != ~0ull && (lane == 63 || next != p)) acc_point<ACC>(m, (int)p, run);
+}
+
 // Wave-aggregated append: every lane of the wave must call it.
 __device__ __forceinline__ unsigned wave_append(unsigned *counter, bool pred)
 {

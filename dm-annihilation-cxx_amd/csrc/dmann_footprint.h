@@ -304,6 +304,56 @@ DMANN_HD RingRec ring_setup(const PGeo &g, int nside, int gi)
     return o;
 }
 
+// Position of ring r in a particle's ring list (north polar cap, partial
+// rings irmin..irmax, south polar cap), as ring_setup counts them; r must
+// lie in the particle's ring range (sky_ring_range).
+DMANN_HD int ring_index(const PGeo &g, int r)
+{
+    if (g.all || r < g.irmin) return r - 1;
+    if (r <= g.irmax) return g.nnorth + (r - g.irmin);
+    return g.nnorth + g.npart + (r - g.irmax - 1);
+}
+
+// First and last ring of a footprint: its ring list is contiguous.
+DMANN_HD void sky_ring_range(const PGeo &g, int nside, int &rlo, int &rhi)
+{
+    rlo = (g.all || g.nnorth > 0) ? 1 : g.irmin;
+    rhi = (g.all || g.nsouth > 0) ? 4 * nside - 1 : g.irmax;
+}
+
+// Ring columns per LDS segment of the sky-covering gather.
+constexpr int kSkySeg = 2048;
+
+// The columns of segment [s0, s0 + ws) of a ring of nr pixels that a
+// footprint with columns c0, c0 + 1, ..., c0 + cnt - 1 (mod nr; c0 in
+// [0, nr), cnt <= nr) covers, as positions t in the footprint's column
+// range: [u, u + na) and, past the wrap, [0, nb).  The segment's columns
+// are positions [u, u + ws) mod nr.
+struct SegSpan {
+    int u, na, nb;
+};
+DMANN_HD SegSpan seg_span(int c0, int cnt, int nr, int s0, int ws)
+{
+    SegSpan o;
+    o.u = s0 - c0;
+    if (o.u < 0) o.u += nr;
+    const int a1 = cnt < o.u + ws ? cnt : o.u + ws;
+    o.na = a1 > o.u ? a1 - o.u : 0;
+    const int b1 = cnt < o.u + ws - nr ? cnt : o.u + ws - nr;
+    o.nb = b1 > 0 ? b1 : 0;
+    return o;
+}
+
+// Slot in segment s0 of footprint position t (t from seg_span).
+DMANN_HD int seg_slot(int c0, int t, int nr, int s0)
+{
+    int col = c0 + t;
+    if (col >= nr) col -= nr;
+    int slot = col - s0;
+    if (slot < 0) slot += nr;
+    return slot;
+}
+
 // Partial ring `iz` of a disc that reaches no pole (the medium class), with
 // the same column range as ring_setup but cheaper: the half-width dphi is
 // first computed with FP32 atan2 from the FP64 terms x and 1 - z^2 - x^2;

@@ -566,13 +566,14 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
             const int BH = bd.BH, Sb = bd.Sb;
             const bool direct = bd.direct;
             const int b1 = b0 + BH - 1;
-            // members meeting the band, in order; the cached ring terms of
-            // the next one are loaded while the current one is walked
-            auto next_member = [&](int j) {
-                for (++j; j < nm; ++j) {
-                    const int lo = wave_read(my_lo, j), hi = wave_read(my_hi, j);
-                    if (wave_read((int)my_on, j) && hi >= b0 && lo <= b1) break;
-                }
+            // members meeting the band, in order (a ballot of them, taken
+            // lowest first); the cached ring terms of the next one are
+            // loaded while the current one is walked
+            uint64_t meet = __ballot(my_on && my_hi >= b0 && my_lo <= b1);
+            auto next_member = [&](int) {
+                if (meet == 0) return nm;
+                const int j = __ffsll((unsigned long long)meet) - 1;
+                meet &= meet - 1;
                 return j;
             };
             auto fetch = [&](int j, float4 &f, int4 &i) {

@@ -59,6 +59,7 @@ struct DevParams {
     double decay_factor, vel_to_cgs, c_in_cgs;
     double cpos[3];
     double footprint_pad; // radians added to 2r for pixel membership
+    int subpixel_sort;    // 1: sub-pixel deposits reduced per wavefront by pixel first
     // halo cores (ANN_CORE_NOSOM): rows (xc, yc, zc, core radius); a uniform
     // grid of cells, each listing the rows whose core meets it (ascending)
     const float4 *halos;
@@ -87,6 +88,11 @@ struct LaunchCfg {
     int prepass_blocks;
     int medium_blocks;
     int large_blocks;
+    // single-wave workgroups of the ring-owned gather of the sky-covering
+    // class (dmann_sky.hip; FP32 and two-float maps with the grouping's
+    // buffers); 0 renders that class with the per-particle kernel
+    // k_disc_large
+    int sky_blocks;
 };
 
 // Grouped rendering of the medium class (dmann_group.hip): counting sort by
@@ -117,6 +123,13 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
                         MapRef map,
                         unsigned long long *stats, hipStream_t st,
                         hipEvent_t sorted = nullptr);
+
+// Sky-covering class by normalisation per particle, then a gather per
+// ring (dmann_sky.hip); acc kAccF32 or kAccComp; ranges: one word per
+// sky-covering particle (the grouping's key buffer serves).
+hipError_t launch_sky(int acc, DiscRec *recs, int64_t n, unsigned *counts, int nside, double pad, MapRef map,
+                      unsigned long long *stats, unsigned *ranges, int norm_blocks, int gather_blocks,
+                      hipStream_t st);
 
 // ev (optional): 4 events recorded before the pre-pass, after it, after the
 // sky-covering kernel and after the medium kernel.

@@ -60,19 +60,23 @@ __global__ __launch_bounds__(256) void k_prepass(const Particle *__restrict__ pa
         for (int j = 0; j < J; ++j) {
             const int64_t i = base + (int64_t)j * blockDim.x + threadIdx.x;
             cls[j] = kClsSkip;
+            rec[j].p0 = 0;
+            rec[j].flux = 0.0;
             if (i < n) {
                 const Particle q = parts[i];
                 bool bad;
                 const int c = prepass_one(P, q, rec[j], bad);
                 if (bad) n_bad++;
                 if (c != kClsSkip) n_kept++;
-                if (c == kClsPoint) {
-                    n_sub++;
-                    acc_point<ACC>(map, rec[j].p0, rec[j].flux);
-                } else if (c == kClsMedium || c == kClsLarge) {
-                    cls[j] = c;
-                }
+                if (c == kClsPoint) n_sub++;
+                cls[j] = c;
             }
+            // sub-pixel particles: the whole flux to their pixel, reduced
+            // over the wavefront by pixel when two or more lanes have one
+            const bool sub = cls[j] == kClsPoint;
+            const uint64_t ms = __ballot(sub);
+            if (P.subpixel_sort && __popcll(ms) >= 2) wave_subpixel_deposit<ACC>(map, sub, rec[j].p0, rec[j].flux);
+            else if (sub) acc_point<ACC>(map, rec[j].p0, rec[j].flux);
             mm[j] = __ballot(cls[j] == kClsMedium);
             ml[j] = __ballot(cls[j] == kClsLarge);
             tm += (unsigned)__popcll(mm[j]);
@@ -717,14 +721,20 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
                                   unsigned *counts, MapRef map, unsigned long long *stats,
                                   const LaunchCfg &cfg, hipStream_t st, hipEvent_t *ev, const GroupCfg *gc)
 {
-    k_fill_u32<<<1, 64, 0, st>>>(counts, 3);
+    k_fill_u32<<<1, 64, 0, st>>>(counts, 4);
     const int64_t want = (n + 256 * kPrepassItems - 1) / (256 * kPrepassItems);
     const int pre_blocks = (int)(want < cfg.prepass_blocks ? want : cfg.prepass_blocks);
     if (ev) (void)hipEventRecord(ev[0], st);
     if (pre_blocks > 0)
         k_prepass<ACC><<<pre_blocks, 256, 0, st>>>(d_parts, n, P, recs, counts, map, stats);
     if (ev) (void)hipEventRecord(ev[1], st);
-    k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, P.footprint_pad, map, stats);
+    if ((ACC == kAccF32 || ACC == kAccComp) && cfg.sky_blocks > 0 && gc && gc->keys) {
+        const hipError_t e = launch_sky(ACC, recs, n, counts, P.nside, P.footprint_pad, map, stats, gc->keys,
+                                        cfg.large_blocks, cfg.sky_blocks, st);
+        if (e != hipSuccess) return e;
+    } else {
+        k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, P.footprint_pad, map, stats);
+    }
     if (ev) (void)hipEventRecord(ev[2], st);
     if ((ACC == kAccF32 || ACC == kAccComp) && gc && gc->k > 0) {
         const hipError_t e =

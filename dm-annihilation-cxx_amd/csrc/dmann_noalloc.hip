@@ -112,6 +112,7 @@ int dmann_render_map_noalloc(const dmann_params *p, const dmann_particle *dev_pa
     cfg.prepass_blocks = cus * 64;
     cfg.medium_blocks = cus * 8;
     cfg.large_blocks = cus * 4;
+    cfg.sky_blocks = 0; // the gather needs the grouping's key buffer
     // clears the output map and the statistics (fill kernels, no memset)
     if ((e = launch_clear(acc, map, npix, stats, st)) != hipSuccess) return nfail(DMANN_ERR_HIP, "clear", e);
     const Particle *parts = reinterpret_cast<const Particle *>(dev_parts);

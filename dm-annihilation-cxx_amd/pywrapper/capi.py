@@ -72,12 +72,14 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_set_stream", "dmann_synchronize", "dmann_get_map", "dmann_get_map_device",
            "dmann_reset", "dmann_get_stats", "dmann_npix", "dmann_destroy",
            "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint", "dmann_host_ring_check",
+           "dmann_host_sky_check",
            "dmann_timing", "dmann_adjoint_device", "dmann_last_error", "dmann_set_tuning", "dmann_set_grouping",
            "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device", "dmann_batch_set_stream",
            "dmann_batch_synchronize", "dmann_batch_npix", "dmann_batch_map_device", "dmann_batch_get_map",
            "dmann_batch_get_stats", "dmann_batch_graph_replays", "dmann_batch_destroy",
            "dmann_batch_last_error", "dmann_read_particles", "dmann_set_halo_cores", "dmann_read_halo_file",
-           "dmann_render_map_noalloc")
+           "dmann_render_map_noalloc", "dmann_set_sky_gather",
+           "dmann_set_subpixel_sort")
 
 _lib = None
 
@@ -124,9 +126,13 @@ def load_library():
     L.dmann_host_footprint.restype = i64
     L.dmann_host_ring_check.argtypes = [C.POINTER(DmannParams), vp]
     L.dmann_host_ring_check.restype = i64
+    L.dmann_host_sky_check.argtypes = [C.POINTER(DmannParams), vp]
+    L.dmann_host_sky_check.restype = i64
     L.dmann_adjoint_device.argtypes = [vp, vp, i64, vp, vp]
     L.dmann_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.dmann_set_grouping.argtypes = [vp, C.c_int, C.c_int]
+    L.dmann_set_sky_gather.argtypes = [vp, C.c_int]
+    L.dmann_set_subpixel_sort.argtypes = [vp, C.c_int]
     L.dmann_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.dmann_last_error.argtypes = [vp]
     L.dmann_last_error.restype = C.c_char_p
@@ -293,6 +299,17 @@ class Projector:
         _check(self._L.dmann_set_tuning(self._h, int(medium_blocks_per_cu), int(large_blocks_per_cu),
                                         int(large_pix)), self._h)
 
+    def set_sky_gather(self, blocks_per_cu: int = 8):
+        """Sky-covering class as a per-ring gather with that many
+        single-wavefront workgroups per CU (0: one workgroup per particle
+        with per-pixel atomics)."""
+        _check(self._L.dmann_set_sky_gather(self._h, int(blocks_per_cu)), self._h)
+
+    def set_subpixel_sort(self, on: bool = True):
+        """Sub-pixel deposits reduced per wavefront by pixel (True) or one
+        atomic per particle (False)."""
+        _check(self._L.dmann_set_subpixel_sort(self._h, 1 if on else 0), self._h)
+
     def set_grouping(self, k: int = 32, nside_c: int | None = None):
         """K neighbours per wavefront (0: one particle per wavefront)."""
         ns = self.params.nside
@@ -412,6 +429,17 @@ def host_ring_check(params: DmannParams, rec) -> int:
     if r.size != 12:
         r = np.ascontiguousarray(np.asarray(rec, dtype=np.float32).reshape(12))
     return int(L.dmann_host_ring_check(C.byref(params), C.c_void_p(r.ctypes.data)))
+
+
+def host_sky_check(params: DmannParams, rec) -> int:
+    """For a sky-covering particle: the number of differences between the
+    pixels and weights the per-ring gather visits and those of the
+    per-particle ring walk (0 expected); -2 for other classes."""
+    L = load_library()
+    r = np.ascontiguousarray(np.asarray(rec).reshape(-1)[:1]).view(np.float32)
+    if r.size != 12:
+        r = np.ascontiguousarray(np.asarray(rec, dtype=np.float32).reshape(12))
+    return int(L.dmann_host_sky_check(C.byref(params), C.c_void_p(r.ctypes.data)))
 
 
 # ------------------------------------------------------------------ batch

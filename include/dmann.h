@@ -148,8 +148,18 @@ int dmann_params_from_ini(const char *ini, int precision, dmann_params *p, int *
  * Replaces initializeCUDA (src/chealpixrender.cu:33-80). */
 int dmann_create(dmann_ctx **ctx, const dmann_params *p, int64_t max_parts, int device);
 
-/* Renders n host-resident particles (copied to the device first) and adds
- * them to the map.  Replaces calculateMapByGPU (src/chealpixrender.cu:83-103). */
+/* Adds n host-resident particles to the map.  Replaces calculateMapByGPU
+ * (src/chealpixrender.cu:83-103).  The particles are copied into pinned
+ * staging memory before the call returns (the caller may reuse its buffer
+ * at once) and rendered in launches of 2^22 particles: small calls -- a
+ * reader's mem_buffer chunks -- are gathered, larger ones split, and the
+ * host-to-device copy of one launch overlaps the render of the previous
+ * one.  A partly filled launch is rendered by the next call that needs
+ * the map or orders work (dmann_synchronize, dmann_get_map,
+ * dmann_get_map_device, dmann_get_stats, dmann_render_device,
+ * dmann_timing, dmann_set_stream, dmann_set_grouping,
+ * dmann_set_halo_cores); dmann_reset drops particles not yet rendered
+ * together with the map.  Errors of the launches surface at those calls. */
 int dmann_render(dmann_ctx *ctx, const dmann_particle *host_parts, int64_t n);
 
 /* Same for particles already resident in device memory.  Asynchronous on the
@@ -162,6 +172,23 @@ int dmann_render_device(dmann_ctx *ctx, const dmann_particle *dev_parts, int64_t
  * (large) footprint kernels, and the estimated footprint size above which a
  * particle goes to the workgroup kernel.  Defaults 8, 4, 4194304. */
 int dmann_set_tuning(dmann_ctx *ctx, int medium_blocks_per_cu, int large_blocks_per_cu, int large_pix);
+
+/* Sub-pixel particles (no counterpart in the reference; the rule is
+ * src/genmap.cpp:106-110): on = 1 sorts each wavefront's sub-pixel
+ * deposits by pixel and adds one FP64 sum per distinct pixel; 0 adds one
+ * atomic per particle.  Default 1. */
+int dmann_set_subpixel_sort(dmann_ctx *ctx, int on);
+
+/* Sky-covering class (no counterpart in the reference): blocks_per_cu >= 1
+ * renders it as a gather -- a normalisation pass per particle, then one
+ * wavefront per HEALPix ring owning that ring's pixels, accumulating up to
+ * 64 particles at a time in LDS and adding them into the map without
+ * atomics -- with that many single-wavefront workgroups per CU; 0 renders
+ * it with one workgroup per particle and per-pixel atomics.  Default 8.
+ * The gather keeps each particle's ring range in the grouping's key
+ * buffer: with grouping off (dmann_set_grouping K = 0), and in the FP64
+ * form, the per-particle kernel is used. */
+int dmann_set_sky_gather(dmann_ctx *ctx, int blocks_per_cu);
 
 /* Grouped rendering of the medium class (no counterpart in the reference):
  * particles are sorted by RING cell at nside_c (1..min(nside, 1024)), then
@@ -307,6 +334,14 @@ int64_t dmann_host_footprint(const dmann_params *p, const dmann_particle *q, int
  * not in the medium class.  A test aid with no counterpart in the
  * reference. */
 int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q);
+
+/* Host-side check of the sky-covering gather's indexing: for a particle of
+ * the sky-covering class, the pixels and FP32 weights that the per-ring
+ * gather visits (ring range, ring position, 2048-column segments, slots)
+ * against those of the per-particle ring walk.  Returns the number of
+ * differences (0 expected), -2 when the particle is not of that class, -1
+ * for bad arguments. */
+int64_t dmann_host_sky_check(const dmann_params *p, const dmann_particle *q);
 
 /* Allocation-free render.  No counterpart in the reference: for callers
  * that cannot allocate device memory -- code under HIP graph capture, tight

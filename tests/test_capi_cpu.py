@@ -110,6 +110,49 @@ def test_ring_lite_matches_ring_setup(pw, tmp_path, nside):
     assert checked > 1000
 
 
+def _placed(pw, p, cases):
+    """Particles at (colatitude, longitude, distance, smoothing length) in
+    the map frame, placed by inverting the observer rotation."""
+    R = np.array(list(p.rotmatrix)).reshape(3, 3)
+    out = []
+    for th, ph, d, h in cases:
+        v = np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+        t = np.array([-v[0], -v[1], v[2]]) * d  # undo the phi - pi shift
+        pos = R @ t + np.array([8.0, 0, 0])
+        rec = np.zeros(1, dtype=pw.PARTICLE_DTYPE)
+        rec["posx"], rec["posy"], rec["posz"] = pos.astype(np.float32)
+        rec["mass"], rec["dens"], rec["hsmooth"] = 4100.0, 1.0, h
+        out.append(rec[0])
+    return out
+
+
+@pytest.mark.parametrize("nside", [64, 512, 1024])
+def test_sky_gather_indexing(pw, tmp_path, nside):
+    """The sky-covering gather (one wavefront per ring, 2048-column LDS
+    segments) visits every pixel of the per-particle ring walk's footprint
+    once, with the same FP32 weight, and no other pixel: footprints over a
+    pole, across phi = 0, whole-sky ones, and at nside 1024 rings of 4096
+    columns (two segments) and polar rings of 2052..4092 (a partial second
+    segment)."""
+    ini = pw.write_ini(str(tmp_path / "o.ini"), nside=nside)
+    p, _, _ = pw.params_from_ini(ini, "fp32")
+    rng = np.random.default_rng(11)
+    cases = [(0.05, 1.0, 1.0, 0.05), (np.pi - 0.03, 4.0, 1.0, 0.04), (1.2, 1e-4, 1.0, 0.3),
+             (2.0, 2 * np.pi - 1e-4, 1.0, 0.35), (1.0, 3.0, 0.2, 0.5), (1.57, 0.5, 0.5, 0.6),
+             (0.4, 2.0, 1.0, 0.8)]
+    for _ in range(12):
+        cases.append((float(np.arccos(rng.uniform(-1, 1))), float(rng.uniform(0, 2 * np.pi)), 1.0,
+                      float(rng.uniform(0.26, 0.9))))
+    checked = 0
+    for q in _placed(pw, p, cases):
+        m = pw.capi.host_sky_check(p, q)
+        if m == -2:
+            continue
+        assert m == 0
+        checked += 1
+    assert checked >= 15
+
+
 def test_host_footprint_edge_geometries(pw, orc, tmp_path):
     """Footprints over a pole, across the phi = 0 seam, covering the whole
     sky, and tiny ones, placed by inverting the observer rotation."""
