@@ -194,7 +194,10 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
     if (!out || !params || nmaps < 1) return bfail(nullptr, DMANN_ERR_ARG, "bad arguments");
     *out = nullptr;
     const int prec = params[0].precision;
-    if (prec < DMANN_PREC_FP32 || prec > DMANN_PREC_FP64) return bfail(nullptr, DMANN_ERR_ARG, "unknown precision");
+    if (prec < DMANN_PREC_FP32 || prec > DMANN_PREC_FP64)
+        return bfail(nullptr, DMANN_ERR_ARG,
+                     prec == DMANN_PREC_FIXED ? "the reproducible precision is available for single maps only"
+                                              : "unknown precision");
     for (int m = 0; m < nmaps; m++) {
         if (params[m].nside < 1 || params[m].nside > kMaxNside)
             return bfail(nullptr, DMANN_ERR_ARG, "map %d: nside %d outside 1..%d", m, params[m].nside, kMaxNside);
@@ -229,7 +232,7 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
     b->cfg.prepass_blocks = b->cus * 64;
     b->cfg.medium_blocks = b->cus * 8;
     b->cfg.large_blocks = b->cus * 4;
-    b->cfg.sky_blocks = b->cus * 8;
+    b->cfg.sky_blocks = b->cus * 12;
     b->gc.k = 32;
     b->gc.sort_blocks = b->cus * 16;
     b->gc.group_blocks = b->cus * DMANN_GROUP_BPC;

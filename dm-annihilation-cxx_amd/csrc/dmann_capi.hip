@@ -36,8 +36,16 @@ struct dmann_ctx {
     DiscRec *d_recs = nullptr;
     unsigned *d_counts = nullptr;
     unsigned long long *d_stats = nullptr;
-    MapRef map{nullptr, nullptr, nullptr};
+    MapRef map{nullptr, nullptr, nullptr, nullptr, 1.0};
     double *d_out = nullptr; // finalised FP64 map for host reads
+    // the reproducible form: its buffers, and the map's fixed-point scale
+    // 2^fix_E, set at the first render after a reset from the kept flux
+    // (fix_total, summed over the renders since) and lowered by shifting
+    // the map when a later render would overflow it
+    DetCfg det{};
+    int fix_E = 0;
+    bool fix_set = false;
+    double fix_total = 0.0;
     LaunchCfg cfg{};
     GroupCfg gc{};
     int cus = 0;
@@ -164,9 +172,48 @@ static void free_stage(dmann_ctx *c)
     c->fill = 0;
 }
 
+static void free_det(dmann_ctx *c)
+{
+    (void)hipFree(c->det.tile_med);
+    (void)hipFree(c->det.tile_large);
+    (void)hipFree(c->det.tile_sums);
+    (void)hipFree(c->det.flux_part);
+    (void)hipFree(c->det.flux_total);
+    (void)hipFree(c->det.keys_out);
+    (void)hipFree(c->det.iota);
+    (void)hipFree(c->det.sort_tmp);
+    c->det = DetCfg{};
+}
+
+// Buffers of the reproducible form for rec_cap records.
+static hipError_t alloc_det(dmann_ctx *c)
+{
+    free_det(c);
+    const int64_t cap = c->rec_cap ? c->rec_cap : 1;
+    const int64_t tiles = (cap + 511) / 512 + 1;
+    const size_t bytes = det_sort_bytes(cap);
+    if (bytes == 0) return hipErrorInvalidValue;
+    hipError_t e;
+    if ((e = hipMalloc(&c->det.tile_med, tiles * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->det.tile_large, tiles * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->det.tile_sums, ((tiles + 1023) / 1024) * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->det.flux_part, (c->cfg.prepass_blocks > 0 ? c->cfg.prepass_blocks : 1) * sizeof(double))) !=
+            hipSuccess ||
+        (e = hipMalloc(&c->det.flux_total, sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&c->det.keys_out, cap * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->det.iota, cap * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->det.sort_tmp, bytes)) != hipSuccess) {
+        free_det(c);
+        return e;
+    }
+    c->det.sort_tmp_bytes = bytes;
+    return hipSuccess;
+}
+
 static void free_all(dmann_ctx *c)
 {
     free_stage(c);
+    free_det(c);
     free_group(c);
     free_halos(c);
     hipFree(c->d_recs);
@@ -175,11 +222,12 @@ static void free_all(dmann_ctx *c)
     hipFree(c->map.f32);
     hipFree(c->map.lo);
     hipFree(c->map.f64);
+    hipFree(c->map.fix);
     hipFree(c->d_out);
     c->d_recs = nullptr;
     c->d_counts = nullptr;
     c->d_stats = nullptr;
-    c->map = MapRef{nullptr, nullptr, nullptr};
+    c->map = MapRef{nullptr, nullptr, nullptr, nullptr, 1.0};
     c->d_out = nullptr;
 }
 
@@ -226,6 +274,7 @@ static int ensure_stage(dmann_ctx *c)
         HIPCHK(c, hipMalloc(&c->d_recs, kStage * sizeof(DiscRec)));
         c->rec_cap = kStage;
         if (k > 0) HIPCHK(c, alloc_group(c, k, nside_c));
+        if (c->acc == kAccFix) HIPCHK(c, alloc_det(c));
     }
     for (int b = 0; b < 2; b++) {
         HIPCHK(c, hipHostMalloc((void **)&c->pin[b], kStage * sizeof(dmann_particle), hipHostMallocDefault));
@@ -234,6 +283,49 @@ static int ensure_stage(dmann_ctx *c)
         HIPCHK(c, hipEventCreateWithFlags(&c->consumed[b], hipEventDisableTiming));
     }
     HIPCHK(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    return DMANN_OK;
+}
+
+// The fixed-point scale of the reproducible form before a render of n
+// particles: their kept flux (a bound on any pixel's share) is summed on
+// the device in a fixed order and read back; the first render after a
+// reset sets the scale 2^E so that the flux so far stays below 2^58 (room
+// for 16 times more in later renders), and a later render that would take
+// the total past 2^61 shifts the map down first.
+static int fix_prepare(dmann_ctx *c, const Particle *parts, int64_t n)
+{
+    const int blocks = c->cfg.prepass_blocks > 0 ? c->cfg.prepass_blocks : 1;
+    HIPCHK(c, launch_flux_total(parts, n, c->P, c->det, blocks, c->stream));
+    double t = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&t, c->det.flux_total, sizeof t, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!(t >= 0.0) || t > 1e300) return fail(c, DMANN_ERR_DOMAIN, "kept flux %g cannot be held in fixed point", t);
+    c->fix_total += t;
+    if (c->fix_total <= 0.0) return DMANN_OK; // nothing deposited yet
+    int ex = 0;
+    (void)frexp(c->fix_total, &ex); // fix_total < 2^ex
+    if (!c->fix_set) {
+        c->fix_E = 58 - ex;
+        c->fix_set = true;
+    } else if (ex + c->fix_E > 61) {
+        const int k = ex + c->fix_E - 58;
+        HIPCHK(c, launch_fix_shift(c->map.fix, c->npix, k, c->stream));
+        c->fix_E -= k;
+    }
+    c->map.fix_scale = ldexp(1.0, c->fix_E);
+    return DMANN_OK;
+}
+
+// Launches one render of n device-resident particles on the context's
+// stream (the reproducible form first fixes the map's scale).
+static int render_launch(dmann_ctx *c, const Particle *parts, int64_t n, hipEvent_t *ev)
+{
+    if (c->acc == kAccFix) {
+        const int r = fix_prepare(c, parts, n);
+        if (r != DMANN_OK) return r;
+    }
+    HIPCHK(c, launch_render(c->acc, parts, n, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev,
+                            &c->gc, c->acc == kAccFix ? &c->det : nullptr));
     return DMANN_OK;
 }
 
@@ -258,8 +350,8 @@ static int stage_flush(dmann_ctx *c)
         }
         ev = &c->events[c->events.size() - 5];
     }
-    HIPCHK(c, launch_render(c->acc, c->dstage[b], c->fill, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg,
-                            c->stream, ev, &c->gc));
+    const int r = render_launch(c, c->dstage[b], c->fill, ev);
+    if (r != DMANN_OK) return r;
     HIPCHK(c, hipEventRecord(c->consumed[b], c->stream));
     c->cur ^= 1;
     c->fill = 0;
@@ -287,7 +379,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     if (max_parts < 0 || max_parts > (int64_t)INT32_MAX * 2)
         return fail(nullptr, DMANN_ERR_ARG, "max_parts %lld outside 0..%lld", (long long)max_parts,
                     (long long)INT32_MAX * 2);
-    if (p->precision < DMANN_PREC_FP32 || p->precision > DMANN_PREC_FP64)
+    if (p->precision < DMANN_PREC_FP32 || p->precision > DMANN_PREC_FIXED)
         return fail(nullptr, DMANN_ERR_ARG, "unknown precision %d", p->precision);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -297,7 +389,10 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     dmann_ctx *c = new (std::nothrow) dmann_ctx();
     if (!c) return fail(nullptr, DMANN_ERR_ARG, "out of host memory");
     c->device = device;
-    c->acc = p->precision == DMANN_PREC_FP32 ? kAccF32 : p->precision == DMANN_PREC_COMPENSATED ? kAccComp : kAccF64;
+    c->acc = p->precision == DMANN_PREC_FP32          ? kAccF32
+             : p->precision == DMANN_PREC_COMPENSATED ? kAccComp
+             : p->precision == DMANN_PREC_FIXED       ? kAccFix
+                                                      : kAccF64;
     c->max_parts = max_parts;
     c->rec_cap = max_parts;
     c->npix = npix_of(p->nside);
@@ -314,7 +409,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     c->cfg.prepass_blocks = dev_cus * 64;
     c->cfg.medium_blocks = dev_cus * 8;
     c->cfg.large_blocks = dev_cus * 4;
-    c->cfg.sky_blocks = dev_cus * 8;
+    c->cfg.sky_blocks = dev_cus * 12;
     auto bail = [&](const char *what, hipError_t err) {
         snprintf(g_create_msg, sizeof g_create_msg, "%s: %s", what, hipGetErrorString(err));
         free_all(c);
@@ -325,6 +420,9 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     const int64_t np = c->npix;
     if (c->acc == kAccF64) {
         if ((e = hipMalloc(&c->map.f64, np * sizeof(double))) != hipSuccess) return bail("hipMalloc map", e);
+    } else if (c->acc == kAccFix) {
+        if ((e = hipMalloc(&c->map.fix, np * sizeof(unsigned long long))) != hipSuccess)
+            return bail("hipMalloc map", e);
     } else {
         if ((e = hipMalloc(&c->map.f32, np * sizeof(float))) != hipSuccess) return bail("hipMalloc map", e);
         if (c->acc == kAccComp && (e = hipMalloc(&c->map.lo, np * sizeof(float))) != hipSuccess)
@@ -338,6 +436,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     c->gc.sort_blocks = dev_cus * 16;
     c->gc.group_blocks = dev_cus * DMANN_GROUP_BPC;
     if ((e = alloc_group(c, 32, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
+    if (c->acc == kAccFix && (e = alloc_det(c)) != hipSuccess) return bail("hipMalloc reproducible-form buffers", e);
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);
     *out = c;
@@ -507,9 +606,7 @@ int dmann_render_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n
         ev = &c->events[c->events.size() - 5];
     }
     if (c->gc.k > 0 && !group_ready(c->gc)) return fail(c, DMANN_ERR_HIP, "grouping buffers are missing");
-    HIPCHK(c, launch_render(c->acc, reinterpret_cast<const Particle *>(dev_parts), n, c->P, c->d_recs,
-                            c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev, &c->gc));
-    return DMANN_OK;
+    return render_launch(c, reinterpret_cast<const Particle *>(dev_parts), n, ev);
 }
 
 int dmann_adjoint_device(dmann_ctx *c, const dmann_particle *dev_parts, int64_t n, const double *dev_grad_map,
@@ -622,6 +719,9 @@ int dmann_reset(dmann_ctx *c)
     if (!c) return DMANN_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     c->fill = 0; // gathered particles not yet rendered are dropped with the map
+    c->fix_set = false;
+    c->fix_total = 0.0;
+    c->map.fix_scale = 1.0;
     HIPCHK(c, launch_clear(c->acc, c->map, c->npix, c->d_stats, c->stream));
     return DMANN_OK;
 }

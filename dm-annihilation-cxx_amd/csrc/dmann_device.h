@@ -107,6 +107,8 @@ __device__ __forceinline__ void acc_add(const MapRef &m, int addr, float v32, do
         if (err != 0.0f) unsafeAtomicAdd(m.lo + addr, err);
     } else if constexpr (ACC == kAccW64) {
         unsafeAtomicAdd(m.f64 + addr, (double)v32);
+    } else if constexpr (ACC == kAccFix) {
+        atomicAdd(m.fix + addr, (unsigned long long)__double2ll_rn((double)v32 * m.fix_scale));
     } else {
         unsafeAtomicAdd(m.f64 + addr, v64);
     }
@@ -127,6 +129,8 @@ __device__ __forceinline__ void acc_point(const MapRef &m, int addr, double flux
         if (err != 0.0f) unsafeAtomicAdd(m.lo + addr, err);
     } else if constexpr (ACC == kAccW64) {
         unsafeAtomicAdd(m.f64 + addr, flux);
+    } else if constexpr (ACC == kAccFix) {
+        atomicAdd(m.fix + addr, (unsigned long long)__double2ll_rn(flux * m.fix_scale));
     } else {
         acc_add<ACC>(m, addr, (float)flux, flux);
     }

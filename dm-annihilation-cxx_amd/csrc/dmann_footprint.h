@@ -108,6 +108,20 @@ DMANN_HD double plugin_flux(const DevParams &P, const Particle &q, double d)
     }
 }
 
+// The filter, the observer-frame offset and distance, and the flux of one
+// particle (src/genmap.cpp:82-98); false when the particle is skipped.
+DMANN_HD bool kept_flux(const DevParams &P, const Particle &q, double &dx, double &dy, double &dz, double &d,
+                        double &flux)
+{
+    if (((double)q.mass >= P.mass_cut) || (q.dens < 0)) return false;
+    dx = (double)q.posx - P.opos[0];
+    dy = (double)q.posy - P.opos[1];
+    dz = (double)q.posz - P.opos[2];
+    d = sqrt(dx * dx + dy * dy + dz * dz);
+    flux = plugin_flux(P, q, d);
+    return true;
+}
+
 // Pre-pass of one particle (src/genmap.cpp:82-110; src/anglefuncs.cpp:15-71;
 // the flux plug-in, plugin_flux).  Returns kClsSkip (filtered),
 // kClsPoint (whole flux to rec.p0), kClsMedium or kClsLarge (footprint).
@@ -115,12 +129,8 @@ DMANN_HD double plugin_flux(const DevParams &P, const Particle &q, double d)
 DMANN_HD int prepass_one(const DevParams &P, const Particle &q, DiscRec &rec, bool &bad)
 {
     bad = false;
-    if (((double)q.mass >= P.mass_cut) || (q.dens < 0)) return kClsSkip;
-    const double dx = (double)q.posx - P.opos[0];
-    const double dy = (double)q.posy - P.opos[1];
-    const double dz = (double)q.posz - P.opos[2];
-    const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    const double flux = plugin_flux(P, q, d);
+    double dx, dy, dz, d, flux;
+    if (!kept_flux(P, q, dx, dy, dz, d, flux)) return kClsSkip;
     const double v[3] = {dx, dy, dz};
     double t[3];
     for (int a = 0; a < 3; a++) {
@@ -167,7 +177,7 @@ struct PGeo {
     double dphi; // widest longitude half-width of the disc, asin(sin(rad)/s0) (pi/2 when it reaches a pole)
     int irmin, irmax, nnorth, npart, nsouth, nrings, p0;
     bool all;
-    int mode;  // 0: fast polynomials, 1: general (sin / asin)
+    int mode;  // 0: fast polynomials, 1: general (sin / asin), 2: sin series + fast asin^2 series
     int pv;    // series lengths of the fast form (fast_variant)
     float K4;  // -(0.5/0.333) * 4 * log2(e) / r^2
 };
@@ -214,7 +224,10 @@ DMANN_HD PGeo particle_geo(const DiscRec &rec, int nside, double pad)
     g.nrings = g.all ? 4 * nside - 1 : g.nnorth + g.npart + g.nsouth;
     const double kc = 0.5 / 0.333; // src/genmap.cpp:143
     g.K4 = (float)(-kc * 4.0 * 1.4426950408889634 / (g.r * g.r));
-    g.mode = fast_geometry(g.rad, g.s0) ? 0 : 1;
+    // the asin^2 series of the fast form holds for psi <= 0.5 whatever the
+    // longitude offsets: footprints of radius <= 0.5 that reach far in
+    // longitude (near a pole) take it with sin(x/2) from its own series
+    g.mode = fast_geometry(g.rad, g.s0) ? 0 : (g.rad <= 0.5 ? 2 : 1);
     g.dphi = asin(fmin(1.0, sin(g.rad) / g.s0));
     g.pv = fast_variant(g.rad, g.dphi);
     return g;
@@ -478,6 +491,9 @@ DMANN_HD float fast_exp2(float x)
 //   B = sin(theta_r) sin(theta);  weight = exp(-0.5 (psi/r)^2 / 0.333)
 //   (src/genmap.cpp:137-143) = exp2(K4 (psi/2)^2).
 // MODE 0 (psi <= 0.5, |x| <= 0.5): (psi/2)^2 = h g(h) by polynomials.
+// MODE 2 (psi <= 0.5, any x in [-pi - step, pi + step]): sin(x/2) from its
+//   Taylor series through the y^13 term (y = x/2, |y| <= pi/2 + step/2:
+//   truncation below 7e-10 relative), squared; then as MODE 0.
 // MODE 1 (any geometry): with hp = sin^2(x/2), cp = cos^2(x/2),
 //   h = A cp + E hp and 1 - h = Ac cp + D hp (E, D = sin^2, cos^2 of
 //   (theta_r+theta)/2; Ac = 1 - A): sums of non-negative terms, so
@@ -496,6 +512,23 @@ DMANN_HD float weight32(float x, float A, float B, float Ac, float D, float K4)
         const float hphi = u * p;
         const float h = fmaf(B, hphi, A);
         // asin^2(sqrt h)/h = sum_n 2^(2n) h^(n-1) / (2 n^2 C(2n,n)), h <= 0.0612
+        float gq = fmaf(h, 128.0f / 2079.0f, 128.0f / 1575.0f);
+        gq = fmaf(h, gq, 4.0f / 35.0f);
+        gq = fmaf(h, gq, 8.0f / 45.0f);
+        gq = fmaf(h, gq, 1.0f / 3.0f);
+        gq = fmaf(h, gq, 1.0f);
+        return fast_exp2(K4 * (h * gq));
+    } else if (MODE == 2) {
+        const float y = 0.5f * x;
+        const float y2 = y * y;
+        float p = fmaf(y2, 1.0f / 6227020800.0f, -1.0f / 39916800.0f);
+        p = fmaf(y2, p, 1.0f / 362880.0f);
+        p = fmaf(y2, p, -1.0f / 5040.0f);
+        p = fmaf(y2, p, 1.0f / 120.0f);
+        p = fmaf(y2, p, -1.0f / 6.0f);
+        p = fmaf(y2, p, 1.0f);
+        const float sh = y * p;
+        const float h = fmaf(B, sh * sh, A);
         float gq = fmaf(h, 128.0f / 2079.0f, 128.0f / 1575.0f);
         gq = fmaf(h, gq, 4.0f / 35.0f);
         gq = fmaf(h, gq, 8.0f / 45.0f);
@@ -588,8 +621,10 @@ DMANN_HD float ring_weight32(const PGeo &g, const RingRec &rr, int t)
 {
     const float kf = (float)(t - rr.kc_rel);
     const float x = fmaf(kf, rr.step, rr.offm);
-    // the sky-covering kernel's forms: full series, or the general form
-    if (g.mode != 0) return weight32<1>(x, rr.A, rr.E, rr.Ac, rr.D, g.K4);
+    // the sky-covering kernel's forms: full series, sin series, or the
+    // general form
+    if (g.mode == 1) return weight32<1>(x, rr.A, rr.E, rr.Ac, rr.D, g.K4);
+    if (g.mode == 2) return weight32<2>(x, rr.A, rr.B, 0.0f, 0.0f, g.K4);
     return weight32<0>(x, rr.A, rr.B, 0.0f, 0.0f, g.K4);
 }
 

@@ -24,6 +24,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "dmann_device.h"
 #include "dmann_footprint.h"
 #include "dmann_geom.h"
@@ -32,11 +35,16 @@
 namespace dmann {
 
 // LDS budget of one wave: 20 KB, so 8 single-wave blocks fit a 160 KB CU
-// (the window takes what the tables leave, in whole kSeg blocks).
+// (the window takes what the tables leave, in whole kSeg blocks: per slot
+// 4 bytes, and per kSeg slots one block-owner byte and one segment-ring
+// byte).
 #ifndef DMANN_GROUP_CAP
-#define DMANN_GROUP_CAP ((15280 * kSeg) / (4 * kSeg + 1) / kSeg * kSeg)
+#define DMANN_GROUP_CAP ((15216 * kSeg) / (4 * kSeg + 2) / kSeg * kSeg)
 #endif
 constexpr int kGroupCapF32 = DMANN_GROUP_CAP; // FP32 window slots
+// segments whose table ring is looked up directly (a window's segments all
+// are: a band ring's row fills at most its columns plus a partial segment)
+constexpr int kSegTab = kGroupCapF32 / kSeg + 64;
 
 struct GroupLDS {
     // ring table of up to 64 rings of one member
@@ -50,7 +58,7 @@ struct GroupLDS {
     int4 bt[65];     // first slot, first window column (wrapped into [0, nr)), nr, first pixel
     int bnc[64];     // window columns
     unsigned char ros[kGroupCapF32 / kSeg]; // band ring owning each block of kSeg slots
-    unsigned char q0[64];                   // table ring of each lane's first segment
+    unsigned char q0[kSegTab];              // table ring of each segment
     float acc[kGroupCapF32];
 };
 static_assert(kGroupCapF32 % kSeg == 0, "window blocks");
@@ -89,20 +97,10 @@ __device__ __forceinline__ RingLite ring_unpack(const float4 &f, const int4 &i)
     return rl;
 }
 
-// ceil(a / b) for 0 <= a < 2^24, 0 < b, from an FP32 quotient corrected by
-// one step either way.
-__device__ __forceinline__ int ceil_div_small(int a, int b, float rb)
-{
-    int q = (int)((float)a * rb);
-    if (q * b < a) ++q;
-    if (q > 0 && (q - 1) * b >= a) --q;
-    return q;
-}
-
 // Fills the LDS ring table from lane q's ring terms (lanes q < n hold ring
-// iz0 + q): segment ranges, the table ring of every lane's first segment
-// and, with band_ring0 >= 0, the window slot terms of the band.  Returns
-// the number of segments.  Every lane must call it.
+// iz0 + q): segment ranges, the table ring of every segment (the first
+// kSegTab of them) and, with band_ring0 >= 0, the window slot terms of the
+// band.  Returns the number of segments.  Every lane must call it.
 __device__ __forceinline__ int ring_table_fill(const RingLite &rl, bool on, int iz0, GroupLDS &S, int lane,
                                                int band_ring0)
 {
@@ -110,13 +108,9 @@ __device__ __forceinline__ int ring_table_fill(const RingLite &rl, bool on, int 
     const int nseg = (cnt + kSeg - 1) / kSeg;
     const int incl = wave_scan_full<DppAdd>(nseg, 0);
     const int NS = wave_read(incl, 63);
-    // lanes whose first segment (lane NS / 64) lies in this ring's range
-    if (nseg > 0) {
-        // (ceil_div_small corrects the approximate reciprocal's quotient)
-        const float rns = __builtin_amdgcn_rcpf((float)NS);
-        const int llo = ceil_div_small((incl - nseg) * 64, NS, rns), lhi = ceil_div_small(incl * 64, NS, rns);
-        for (int l = llo; l < lhi; ++l) S.q0[l] = (unsigned char)lane;
-    }
+    // this ring's segments in the segment-ring table
+    const int qe = incl < kSegTab ? incl : kSegTab;
+    for (int l = incl - nseg; l < qe; ++l) S.q0[l] = (unsigned char)lane;
     if (on) {
         S.rf[lane] = make_float4(rl.e0, rl.e1, rl.e2, rl.step);
         S.rx[lane] = rl.offm;
@@ -174,9 +168,13 @@ __device__ __forceinline__ float seg_mask(float lenf, int t)
     return __builtin_amdgcn_fmed3f(lenf - (float)t, 0.0f, 1.0f);
 }
 
-// Walks the table's segments: lane L takes a contiguous run of segments
-// (NS/64 each), kSeg pixels per segment with the ring terms loaded once per
-// segment.  Every lane evaluates all kSeg pixels of its segment; those past
+// Walks the table's segments round-robin: in iteration i lane L takes
+// segment 64 i + L, so the 64 lanes hold consecutive segments -- within a
+// ring their window slots are kSeg (odd) apart, in distinct LDS banks, and
+// their ring-term reads mostly hit one table row (a broadcast).  kSeg
+// pixels per segment with the ring terms loaded once per segment, the
+// segment's table ring from the segment-ring table (beyond it, found by
+// stepping from the last tabled ring).  Every lane evaluates all kSeg pixels of its segment; those past
 // the ring's end are masked to zero (a factor 0 in an fma: the weights
 // there are finite), so the loop has no divergent branch.  Pixel 0 of a
 // segment always lies on the ring.
@@ -189,15 +187,19 @@ __device__ __forceinline__ float seg_mask(float lenf, int t)
 template <int MODE, int ACC, bool WINDOW, int PV>
 __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, float K4, float sc, const MapRef &map)
 {
-    const int s0 = (lane * NS) >> 6, s1 = ((lane + 1) * NS) >> 6;
     const int nmax = (NS + 63) >> 6;
     double total = 0.0;
     float part = 0.0f;
-    int q = s0 < s1 ? S.q0[lane] : 0;
+    int q = 0;
     for (int i = 0; i < nmax; ++i) {
-        const int s = s0 + i;
-        if (s < s1) {
-            while (S.ri[q].w <= s) ++q;
+        const int s = 64 * i + lane;
+        if (s < NS) {
+            if (s < kSegTab) {
+                q = S.q0[s];
+            } else {
+                if (q < S.q0[kSegTab - 1]) q = S.q0[kSegTab - 1];
+                while (S.ri[q].w <= s) ++q;
+            }
             const float4 f = S.rf[q];
             const int4 r = S.ri[q];
             const int t0 = (s - r.z) * kSeg;
@@ -640,20 +642,40 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
 
 // ---------------------------------------------------------------- sorting
 
-// Sort key of every medium-class record -- coarse sky cell (RING pixel at
-// nside_c) times kSizeBuckets plus the footprint-size class -- and the keys'
-// histogram.
+// Sort key of a medium-class record: coarse sky cell (RING pixel at
+// nside_c) times kSizeBuckets plus the footprint-size class.
+__device__ __forceinline__ unsigned sort_key(const DiscRec &r, int nside_c)
+{
+    const int lg = r.est > 0 ? 31 - __clz(r.est) : 0;
+    const int bucket = lg < 6 ? 0 : (lg - 6 > kSizeBuckets - 1 ? kSizeBuckets - 1 : lg - 6);
+    return (unsigned)ang2pix(nside_c, r.theta, r.phi) * (unsigned)kSizeBuckets + (unsigned)bucket;
+}
+
+// Sort key of every medium-class record and the keys' histogram.
 __global__ void k_sort_keys(const DiscRec *__restrict__ recs, const unsigned *__restrict__ counts, int nside_c,
                             unsigned *__restrict__ keys, unsigned *__restrict__ hist)
 {
     const unsigned n = counts[0];
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const DiscRec r = recs[i];
-        const int lg = r.est > 0 ? 31 - __clz(r.est) : 0;
-        const int bucket = lg < 6 ? 0 : (lg - 6 > kSizeBuckets - 1 ? kSizeBuckets - 1 : lg - 6);
-        const unsigned key = (unsigned)ang2pix(nside_c, r.theta, r.phi) * (unsigned)kSizeBuckets + (unsigned)bucket;
+        const unsigned key = sort_key(recs[i], nside_c);
         keys[i] = key;
         atomicAdd(&hist[key], 1u);
+    }
+}
+
+// Keys of the reproducible form's stable radix sort: records below the
+// medium count get their key, the rest of the nrec slots one past the
+// largest key (they sort last), and each slot its own index as the value,
+// so the sorted values list the medium records by key and, within a key,
+// in record order.
+__global__ void k_sort_keys_det(const DiscRec *__restrict__ recs, const unsigned *__restrict__ counts, int nside_c,
+                                int64_t nrec, unsigned pad_key, unsigned *__restrict__ keys,
+                                unsigned *__restrict__ iota)
+{
+    const unsigned n = counts[0];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (int64_t)gridDim.x * blockDim.x) {
+        keys[i] = i < (int64_t)n ? sort_key(recs[i], nside_c) : pad_key;
+        iota[i] = (unsigned)i;
     }
 }
 
@@ -728,23 +750,65 @@ __global__ void k_zero_u32(unsigned *p, int n)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
+// Bits of the reproducible form's sort keys at nside_c: the largest key
+// is the pad key 12 nside_c^2 kSizeBuckets.
+static int det_key_bits(int nside_c)
+{
+    const unsigned top = 12u * (unsigned)nside_c * (unsigned)nside_c * (unsigned)kSizeBuckets;
+    return 32 - __builtin_clz(top);
+}
+
+size_t det_sort_bytes(int64_t n)
+{
+    size_t bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
+                                  (const unsigned *)nullptr, (unsigned *)nullptr, (size_t)(n > 0 ? n : 1), 0,
+                                  det_key_bits(kMaxSortNside)) != hipSuccess)
+        return 0;
+    return bytes;
+}
+
+hipError_t launch_scan_u32(unsigned *data, int n, unsigned *sums, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + 1023) / 1024;
+    k_scan_blocks<<<nb, 256, 0, st>>>(data, n, sums);
+    k_scan_sums<<<1, 256, 0, st>>>(sums, nb);
+    k_scan_add<<<nb, 256, 0, st>>>(data, n, sums);
+    return hipGetLastError();
+}
+
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, double pad, const GroupCfg &gc,
                         MapRef map,
-                        unsigned long long *stats, hipStream_t st, hipEvent_t sorted)
+                        unsigned long long *stats, hipStream_t st, hipEvent_t sorted, const DetCfg *det, int64_t nrec)
 {
     k_zero_u32<<<1, 64, 0, st>>>(gc.work, 1);
     const int nbins = 12 * gc.nside_c * gc.nside_c * kSizeBuckets;
-    const int nb = (nbins + 1023) / 1024;
-    k_zero_u32<<<nb < 4096 ? nb : 4096, 256, 0, st>>>(gc.hist, nbins);
-    k_sort_keys<<<gc.sort_blocks, 256, 0, st>>>(recs, counts, gc.nside_c, gc.keys, gc.hist);
-    k_scan_blocks<<<nb, 256, 0, st>>>(gc.hist, nbins, gc.sums);
-    k_scan_sums<<<1, 256, 0, st>>>(gc.sums, nb);
-    k_scan_add<<<nb, 256, 0, st>>>(gc.hist, nbins, gc.sums);
-    k_sort_scatter<<<gc.sort_blocks, 256, 0, st>>>(gc.keys, counts, gc.hist, gc.idx);
+    if (det) {
+        // stable radix sort of all nrec slots (the medium count is known on
+        // the device only): medium records by key, then by record order
+        k_sort_keys_det<<<gc.sort_blocks, 256, 0, st>>>(recs, counts, gc.nside_c, nrec, (unsigned)nbins, gc.keys,
+                                                          det->iota);
+        size_t bytes = det->sort_tmp_bytes;
+        const hipError_t e = rocprim::radix_sort_pairs(det->sort_tmp, bytes, gc.keys, det->keys_out, det->iota, gc.idx,
+                                                       (size_t)nrec, 0, det_key_bits(gc.nside_c), st);
+        if (e != hipSuccess) return e;
+    } else {
+        const int nb = (nbins + 1023) / 1024;
+        k_zero_u32<<<nb < 4096 ? nb : 4096, 256, 0, st>>>(gc.hist, nbins);
+        k_sort_keys<<<gc.sort_blocks, 256, 0, st>>>(recs, counts, gc.nside_c, gc.keys, gc.hist);
+        k_scan_blocks<<<nb, 256, 0, st>>>(gc.hist, nbins, gc.sums);
+        k_scan_sums<<<1, 256, 0, st>>>(gc.sums, nb);
+        k_scan_add<<<nb, 256, 0, st>>>(gc.hist, nbins, gc.sums);
+        k_sort_scatter<<<gc.sort_blocks, 256, 0, st>>>(gc.keys, counts, gc.hist, gc.idx);
+    }
     if (sorted) (void)hipEventRecord(sorted, st);
     if (acc == kAccComp)
         k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, pad, gc.k, map, stats, gc.work,
                                                                gc.tab_f, gc.tab_i);
+    else if (acc == kAccFix)
+        k_disc_group<kAccFix><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, pad, gc.k, map, stats, gc.work,
+                                                              gc.tab_f, gc.tab_i);
     else
         k_disc_group<kAccF32><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, pad, gc.k, map, stats, gc.work,
                                                               gc.tab_f, gc.tab_i);

@@ -18,9 +18,11 @@ struct Particle {
 static_assert(sizeof(Particle) == 48, "DMParticle is 12 floats");
 
 // Accumulation forms: FP32 weights into an FP32 map, into a two-float map,
-// FP64 weights into an FP64 map, and FP32 weights into an FP64 map (FP64
-// atomics; the allocation-free entry point accumulates into its output).
-enum { kAccF32 = 0, kAccComp = 1, kAccF64 = 2, kAccW64 = 3 };
+// FP64 weights into an FP64 map, FP32 weights into an FP64 map (FP64
+// atomics; the allocation-free entry point accumulates into its output),
+// and FP32 weights into a 64-bit fixed-point map (the reproducible form:
+// integer additions commute, so the map does not depend on their order).
+enum { kAccF32 = 0, kAccComp = 1, kAccF64 = 2, kAccW64 = 3, kAccFix = 4 };
 enum { kClsSkip = 0, kClsPoint = 1, kClsMedium = 2, kClsLarge = 3 };
 enum {
     kStatKept = 0,
@@ -82,7 +84,37 @@ struct MapRef {
     float *f32;  // FP32 map, or the hi word of the two-float map
     float *lo;   // lo word of the two-float map
     double *f64; // FP64 map
+    unsigned long long *fix; // fixed-point map: value x 2^E as a two's-complement int64
+    double fix_scale;        // 2^E
 };
+
+// Buffers of the reproducible form (kAccFix): the pre-pass places records
+// by per-tile offsets (tiles of 512 particles in particle order) instead of
+// atomic appends, and the medium class is sorted by a stable radix sort.
+struct DetCfg {
+    unsigned *tile_med;    // ceil(max_parts / 512): per-tile counts, then exclusive offsets
+    unsigned *tile_large;
+    unsigned *tile_sums;   // scan block sums, ceil(tiles / 1024)
+    double *flux_part;     // per pre-pass block FP64 flux sums
+    double *flux_total;    // their sum (one double)
+    unsigned *keys_out;    // sort output keys, max_parts
+    unsigned *iota;        // sort input values (record indices), max_parts
+    void *sort_tmp;        // radix sort temporary storage
+    size_t sort_tmp_bytes;
+};
+
+// Bytes of radix-sort temporary storage for n keys (dmann_group.hip).
+size_t det_sort_bytes(int64_t n);
+
+// Sum of the kept particles' fluxes (FP64, fixed reduction order) into
+// det.flux_total: a bound on every pixel of the map, which sets its
+// fixed-point scale.
+hipError_t launch_flux_total(const Particle *d_parts, int64_t n, const DevParams &P, const DetCfg &det, int blocks,
+                             hipStream_t st);
+// Fixed-point map values shifted right by k bits (a smaller scale).
+hipError_t launch_fix_shift(unsigned long long *fix, int64_t npix, int k, hipStream_t st);
+// Exclusive scan of n unsigned values in place (sums: ceil(n / 1024)).
+hipError_t launch_scan_u32(unsigned *data, int n, unsigned *sums, hipStream_t st);
 
 struct LaunchCfg {
     int prepass_blocks;
@@ -122,10 +154,10 @@ struct GroupCfg {
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, double pad, const GroupCfg &gc,
                         MapRef map,
                         unsigned long long *stats, hipStream_t st,
-                        hipEvent_t sorted = nullptr);
+                        hipEvent_t sorted = nullptr, const DetCfg *det = nullptr, int64_t nrec = 0);
 
 // Sky-covering class by normalisation per particle, then a gather per
-// ring (dmann_sky.hip); acc kAccF32 or kAccComp; ranges: one word per
+// ring (dmann_sky.hip); acc kAccF32, kAccComp or kAccFix; ranges: one word per
 // sky-covering particle (the grouping's key buffer serves).
 hipError_t launch_sky(int acc, DiscRec *recs, int64_t n, unsigned *counts, int nside, double pad, MapRef map,
                       unsigned long long *stats, unsigned *ranges, int norm_blocks, int gather_blocks,
@@ -133,9 +165,11 @@ hipError_t launch_sky(int acc, DiscRec *recs, int64_t n, unsigned *counts, int n
 
 // ev (optional): 4 events recorded before the pre-pass, after it, after the
 // sky-covering kernel and after the medium kernel.
+// det: the reproducible form's buffers (required for kAccFix).
 hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                          unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
-                         hipStream_t st, hipEvent_t *ev = nullptr, const GroupCfg *gc = nullptr);
+                         hipStream_t st, hipEvent_t *ev = nullptr, const GroupCfg *gc = nullptr,
+                         const DetCfg *det = nullptr);
 hipError_t launch_adjoint(const Particle *d_parts, int64_t n, const DevParams &P, const double *gmap, double *out,
                           int blocks, hipStream_t st);
 hipError_t launch_batch(int acc, const Particle *d_parts, int64_t n, const DevParams *Ps, const MapRef *maps,

@@ -33,11 +33,17 @@ constexpr int kPrepassItems = 2;
 
 // One thread per particle (grid-stride).  Mirrors src/genmap.cpp:82-110 with
 // the helpers of src/anglefuncs.cpp:15-71 and src/flux/flux_ann_nosom.cpp:12-23.
+// In the reproducible form (kAccFix) each tile of 512 particles places its
+// records at the offsets tile_med / tile_large hold (exclusive prefix sums
+// of k_prepass_count's per-tile counts), so the record order follows the
+// particle order whatever the blocks' timing.
 template <int ACC>
 __global__ __launch_bounds__(256) void k_prepass(const Particle *__restrict__ parts, int64_t n,
                                                  DevParams P, DiscRec *__restrict__ recs,
                                                  unsigned *__restrict__ counts, MapRef map,
-                                                 unsigned long long *__restrict__ stats)
+                                                 unsigned long long *__restrict__ stats,
+                                                 const unsigned *__restrict__ tile_med,
+                                                 const unsigned *__restrict__ tile_large)
 {
     // Each block takes kPrepassItems x 256 particles per iteration (item j
     // of thread t is particle base + j 256 + t, so loads stay coalesced)
@@ -45,6 +51,7 @@ __global__ __launch_bounds__(256) void k_prepass(const Particle *__restrict__ pa
     // atomic each: the two append counters are single addresses, and one
     // atomic per wave and 64 particles made them the bottleneck.
     constexpr int J = kPrepassItems;
+    constexpr bool DET = ACC == kAccFix;
     __shared__ unsigned s_cnt[2][4], s_base[2];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -90,7 +97,14 @@ __global__ __launch_bounds__(256) void k_prepass(const Particle *__restrict__ pa
         if (threadIdx.x < 2) {
             const unsigned t = s_cnt[threadIdx.x][0] + s_cnt[threadIdx.x][1] + s_cnt[threadIdx.x][2] +
                                s_cnt[threadIdx.x][3];
-            s_base[threadIdx.x] = t ? atomicAdd(&counts[threadIdx.x], t) : 0u;
+            if constexpr (DET) {
+                // the tile's place in particle order; the counters keep the totals
+                const int64_t tile = base / ((int64_t)blockDim.x * J);
+                s_base[threadIdx.x] = (threadIdx.x == 0 ? tile_med : tile_large)[tile];
+                if (t) atomicAdd(&counts[threadIdx.x], t);
+            } else {
+                s_base[threadIdx.x] = t ? atomicAdd(&counts[threadIdx.x], t) : 0u;
+            }
         }
         __syncthreads();
         unsigned om = s_base[0], ol = s_base[1];
@@ -110,6 +124,82 @@ __global__ __launch_bounds__(256) void k_prepass(const Particle *__restrict__ pa
     wave_count(&stats[kStatKept], n_kept);
     wave_count(&stats[kStatSub], n_sub);
     wave_count(&stats[kStatBad], n_bad);
+}
+
+// Records per tile of the pre-pass (the reproducible form's first pass):
+// the same classification, no deposits; tile_med[t] / tile_large[t] = the
+// medium / sky-covering records of particles t 512 .. t 512 + 511.
+__global__ __launch_bounds__(256) void k_prepass_count(const Particle *__restrict__ parts, int64_t n, DevParams P,
+                                                       unsigned *__restrict__ tile_med,
+                                                       unsigned *__restrict__ tile_large)
+{
+    constexpr int J = kPrepassItems;
+    __shared__ unsigned s_cnt[2][4];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * J;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x * J; base < n; base += stride) {
+        unsigned tm = 0, tl = 0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int64_t i = base + (int64_t)j * blockDim.x + threadIdx.x;
+            int c = kClsSkip;
+            if (i < n) {
+                DiscRec rec;
+                bool bad;
+                c = prepass_one(P, parts[i], rec, bad);
+            }
+            tm += (unsigned)__popcll(__ballot(c == kClsMedium));
+            tl += (unsigned)__popcll(__ballot(c == kClsLarge));
+        }
+        if (lane == 0) {
+            s_cnt[0][wid] = tm;
+            s_cnt[1][wid] = tl;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            const unsigned t = s_cnt[threadIdx.x][0] + s_cnt[threadIdx.x][1] + s_cnt[threadIdx.x][2] +
+                               s_cnt[threadIdx.x][3];
+            (threadIdx.x == 0 ? tile_med : tile_large)[base / ((int64_t)blockDim.x * J)] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// Sum of the kept particles' fluxes: per block in a fixed order (each
+// thread over its grid-stride particles, then a tree over the block), then
+// over the blocks in index order, so the total is the same on every run.
+__global__ __launch_bounds__(256) void k_flux_part(const Particle *__restrict__ parts, int64_t n, DevParams P,
+                                                   double *__restrict__ part)
+{
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double dx, dy, dz, d, flux;
+        if (kept_flux(P, parts[i], dx, dy, dz, d, flux) && flux > 0.0) acc += flux;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void k_flux_sum(const double *__restrict__ part, int nb, double *__restrict__ total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double t = 0.0;
+        for (int b = 0; b < nb; ++b) t += part[b];
+        *total = t;
+    }
+}
+
+__global__ void k_fix_shift(unsigned long long *fix, int64_t npix, int k)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (int64_t)gridDim.x * blockDim.x)
+        fix[i] = (unsigned long long)((long long)fix[i] >> k);
 }
 
 // ---------------------------------------------------------------- footprint
@@ -173,6 +263,8 @@ __device__ __forceinline__ float wt32(float x, const float4 &f, const WinLDS &L,
 {
     if constexpr (MODE == 0) {
         return weight32<0>(x, f.x, f.y, 0.0f, 0.0f, K4);
+    } else if constexpr (MODE == 2) {
+        return weight32<2>(x, f.x, f.y, 0.0f, 0.0f, K4);
     } else {
         const float4 q = L.gq[j];
         return weight32<1>(x, f.x, q.x, q.y, q.z, K4);
@@ -409,6 +501,7 @@ __global__ __launch_bounds__(256) void k_disc_large(const DiscRec *__restrict__ 
                                 const float kf = (float)(k - kref);
                                 float w;
                                 if (g.mode == 0) w = weight32<0>(fmaf(kf, f.w, f.z), f.x, f.y, 0.0f, 0.0f, g.K4);
+                                else if (g.mode == 2) w = weight32<2>(fmaf(kf, f.w, f.z), f.x, f.y, 0.0f, 0.0f, g.K4);
                                 else w = weight32<1>(fmaf(kf, f.w, f.z), f.x, gq.x, gq.y, gq.z, g.K4);
                                 if (pass == 0) part += w;
                                 else acc_add<ACC>(map, pix_addr(a, k), w * (float)scale64, 0.0);
@@ -630,8 +723,9 @@ __global__ __launch_bounds__(256) void k_batch(const Particle *__restrict__ part
                 continue;
             }
             const PGeo g = particle_geo(rec, P.nside, P.footprint_pad);
-            const int T = g.mode == 0 ? render_particle_wave<ACC, 0>(g, P.nside, L, lane, map)
-                                      : render_particle_wave<ACC, 1>(g, P.nside, L, lane, map);
+            const int T = g.mode == 0   ? render_particle_wave<ACC, 0>(g, P.nside, L, lane, map)
+                          : g.mode == 2 ? render_particle_wave<ACC, 2>(g, P.nside, L, lane, map)
+                                        : render_particle_wave<ACC, 1>(g, P.nside, L, lane, map);
             if (T < 2) c_fb++;
             else {
                 c_ext++;
@@ -709,6 +803,7 @@ __global__ void k_finalize(MapRef map, int64_t npix, double dOmega, double *__re
         double v;
         if constexpr (ACC == kAccF32) v = (double)map.f32[i];
         else if constexpr (ACC == kAccComp) v = (double)map.f32[i] + (double)map.lo[i];
+        else if constexpr (ACC == kAccFix) v = (double)(long long)map.fix[i] / map.fix_scale;
         else v = map.f64[i]; // FP64 and W64 maps; in place when out == map.f64
         out[i] = v / dOmega;
     }
@@ -719,16 +814,32 @@ __global__ void k_finalize(MapRef map, int64_t npix, double dOmega, double *__re
 template <int ACC>
 static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                                   unsigned *counts, MapRef map, unsigned long long *stats,
-                                  const LaunchCfg &cfg, hipStream_t st, hipEvent_t *ev, const GroupCfg *gc)
+                                  const LaunchCfg &cfg, hipStream_t st, hipEvent_t *ev, const GroupCfg *gc,
+                                  const DetCfg *det)
 {
+    constexpr bool DET = ACC == kAccFix;
+    constexpr bool SERIES = ACC == kAccF32 || ACC == kAccComp || ACC == kAccFix;
+    if (DET && !det) return hipErrorInvalidValue;
     k_fill_u32<<<1, 64, 0, st>>>(counts, 4);
     const int64_t want = (n + 256 * kPrepassItems - 1) / (256 * kPrepassItems);
     const int pre_blocks = (int)(want < cfg.prepass_blocks ? want : cfg.prepass_blocks);
     if (ev) (void)hipEventRecord(ev[0], st);
+    const unsigned *tmed = nullptr, *tlarge = nullptr;
+    if (DET && pre_blocks > 0) {
+        // records placed in particle order: per-tile counts, their
+        // exclusive prefix sums, then the pre-pass proper
+        const int ntiles = (int)want;
+        k_prepass_count<<<pre_blocks, 256, 0, st>>>(d_parts, n, P, det->tile_med, det->tile_large);
+        hipError_t e;
+        if ((e = launch_scan_u32(det->tile_med, ntiles, det->tile_sums, st)) != hipSuccess) return e;
+        if ((e = launch_scan_u32(det->tile_large, ntiles, det->tile_sums, st)) != hipSuccess) return e;
+        tmed = det->tile_med;
+        tlarge = det->tile_large;
+    }
     if (pre_blocks > 0)
-        k_prepass<ACC><<<pre_blocks, 256, 0, st>>>(d_parts, n, P, recs, counts, map, stats);
+        k_prepass<ACC><<<pre_blocks, 256, 0, st>>>(d_parts, n, P, recs, counts, map, stats, tmed, tlarge);
     if (ev) (void)hipEventRecord(ev[1], st);
-    if ((ACC == kAccF32 || ACC == kAccComp) && cfg.sky_blocks > 0 && gc && gc->keys) {
+    if (SERIES && cfg.sky_blocks > 0 && gc && gc->keys) {
         const hipError_t e = launch_sky(ACC, recs, n, counts, P.nside, P.footprint_pad, map, stats, gc->keys,
                                         cfg.large_blocks, cfg.sky_blocks, st);
         if (e != hipSuccess) return e;
@@ -736,9 +847,9 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
         k_disc_large<ACC><<<cfg.large_blocks, 256, 0, st>>>(recs, n, counts, P.nside, P.footprint_pad, map, stats);
     }
     if (ev) (void)hipEventRecord(ev[2], st);
-    if ((ACC == kAccF32 || ACC == kAccComp) && gc && gc->k > 0) {
-        const hipError_t e =
-            launch_group(ACC, recs, counts, P.nside, P.footprint_pad, *gc, map, stats, st, ev ? ev[3] : nullptr);
+    if (SERIES && gc && gc->k > 0) {
+        const hipError_t e = launch_group(ACC, recs, counts, P.nside, P.footprint_pad, *gc, map, stats, st,
+                                          ev ? ev[3] : nullptr, DET ? det : nullptr, n);
         if (e != hipSuccess) return e;
     } else {
         if (ev) (void)hipEventRecord(ev[3], st);
@@ -750,14 +861,31 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
 
 hipError_t launch_render(int acc, const Particle *d_parts, int64_t n, const DevParams &P, DiscRec *recs,
                          unsigned *counts, MapRef map, unsigned long long *stats, const LaunchCfg &cfg,
-                         hipStream_t st, hipEvent_t *ev, const GroupCfg *gc)
+                         hipStream_t st, hipEvent_t *ev, const GroupCfg *gc, const DetCfg *det)
 {
     switch (acc) {
-    case kAccF32: return launch_render_t<kAccF32>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
-    case kAccComp: return launch_render_t<kAccComp>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
-    case kAccW64: return launch_render_t<kAccW64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, nullptr);
-    default: return launch_render_t<kAccF64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc);
+    case kAccF32: return launch_render_t<kAccF32>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc, nullptr);
+    case kAccComp: return launch_render_t<kAccComp>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc, nullptr);
+    case kAccW64:
+        return launch_render_t<kAccW64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, nullptr, nullptr);
+    case kAccFix: return launch_render_t<kAccFix>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc, det);
+    default: return launch_render_t<kAccF64>(d_parts, n, P, recs, counts, map, stats, cfg, st, ev, gc, nullptr);
     }
+}
+
+hipError_t launch_flux_total(const Particle *d_parts, int64_t n, const DevParams &P, const DetCfg &det, int blocks,
+                             hipStream_t st)
+{
+    k_flux_part<<<blocks, 256, 0, st>>>(d_parts, n, P, det.flux_part);
+    k_flux_sum<<<1, 64, 0, st>>>(det.flux_part, blocks, det.flux_total);
+    return hipGetLastError();
+}
+
+hipError_t launch_fix_shift(unsigned long long *fix, int64_t npix, int k, hipStream_t st)
+{
+    const int blocks = (int)((npix + 255) / 256 < 8192 ? (npix + 255) / 256 : 8192);
+    k_fix_shift<<<blocks, 256, 0, st>>>(fix, npix, k);
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize(int acc, MapRef map, int64_t npix, double dOmega, double *out, hipStream_t st)
@@ -766,6 +894,7 @@ hipError_t launch_finalize(int acc, MapRef map, int64_t npix, double dOmega, dou
     switch (acc) {
     case kAccF32: k_finalize<kAccF32><<<blocks, 256, 0, st>>>(map, npix, dOmega, out); break;
     case kAccComp: k_finalize<kAccComp><<<blocks, 256, 0, st>>>(map, npix, dOmega, out); break;
+    case kAccFix: k_finalize<kAccFix><<<blocks, 256, 0, st>>>(map, npix, dOmega, out); break;
     default: k_finalize<kAccF64><<<blocks, 256, 0, st>>>(map, npix, dOmega, out); break;
     }
     return hipGetLastError();
@@ -775,6 +904,7 @@ hipError_t launch_clear(int acc, MapRef map, int64_t npix, unsigned long long *s
 {
     const int blocks = (int)((npix + 255) / 256 < 8192 ? (npix + 255) / 256 : 8192);
     if (acc == kAccF64 || acc == kAccW64) k_fill_f64<<<blocks, 256, 0, st>>>(map.f64, npix);
+    else if (acc == kAccFix) k_fill_u64n<<<blocks, 256, 0, st>>>(map.fix, npix);
     else k_fill_f32<<<blocks, 256, 0, st>>>(map.f32, npix);
     if (acc == kAccComp) k_fill_f32<<<blocks, 256, 0, st>>>(map.lo, npix);
     k_fill_u64<<<1, 64, 0, st>>>(stats, kNumStats);

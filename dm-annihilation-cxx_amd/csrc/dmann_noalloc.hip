@@ -84,7 +84,8 @@ int dmann_render_map_noalloc(const dmann_params *p, const dmann_particle *dev_pa
         return nfail(DMANN_ERR_ARG, "dmann_render_map_noalloc: null pointer or negative particle count");
     if (p->nside < 1 || p->nside > kMaxNside) return nfail(DMANN_ERR_ARG, "dmann_render_map_noalloc: nside out of range");
     if (p->precision < DMANN_PREC_FP32 || p->precision > DMANN_PREC_FP64)
-        return nfail(DMANN_ERR_ARG, "dmann_render_map_noalloc: unknown precision");
+        return nfail(DMANN_ERR_ARG, "dmann_render_map_noalloc: unknown precision (the reproducible precision "
+                                    "needs a context)");
     // (the halo-core plug-in needs a context's core table)
     if (p->flux_kind < DMANN_FLUX_ANN_NOSOM || p->flux_kind > DMANN_FLUX_ANN_ADIABATIC)
         return nfail(DMANN_ERR_ARG, "dmann_render_map_noalloc: flux plug-in not available without a context");

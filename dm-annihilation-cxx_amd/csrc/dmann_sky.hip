@@ -56,6 +56,29 @@ struct NormLDS {
     int wn[4];
 };
 
+// This lane's share of the weights of one ring row (cnt = c.x pixels, the
+// column nearest the particle at c.y): four columns per lane at a time in
+// four FP32 partial sums while the wave has 256 columns left (at most 32
+// terms each for rings of 8192 pixels), then one at a time.  B2 is B for
+// the series forms and E for the general form.
+template <int MODE>
+__device__ __forceinline__ double row_weights(const int2 &c, const float4 &f, float B2, const float4 &q, float K4,
+                                              int lane)
+{
+    float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int t0 = 0;
+    for (; t0 + 256 <= c.x; t0 += 256) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int t = t0 + 64 * k + lane;
+            part[k] += weight32<MODE>(fmaf((float)(t - c.y), f.w, f.z), f.x, B2, q.y, q.z, K4);
+        }
+    }
+    for (int t = t0 + lane; t < c.x; t += 64)
+        part[0] += weight32<MODE>(fmaf((float)(t - c.y), f.w, f.z), f.x, B2, q.y, q.z, K4);
+    return (double)((part[0] + part[1]) + (part[2] + part[3]));
+}
+
 template <int ACC>
 __global__ __launch_bounds__(256) void k_sky_norm(DiscRec *__restrict__ recs, int64_t n,
                                                   unsigned *__restrict__ counts, int nside, double pad, MapRef map,
@@ -123,13 +146,9 @@ __global__ __launch_bounds__(256) void k_sky_norm(DiscRec *__restrict__ recs, in
                     if (c.x <= 0) continue;
                     const float4 f = S.f[wid][j];
                     const float4 q = S.q[wid][j];
-                    float part = 0.0f;
-                    for (int t = lane; t < c.x; t += 64) {
-                        const float x = fmaf((float)(t - c.y), f.w, f.z);
-                        part += g.mode == 0 ? weight32<0>(x, f.x, f.y, 0.0f, 0.0f, g.K4)
-                                            : weight32<1>(x, f.x, q.x, q.y, q.z, g.K4);
-                    }
-                    nacc += (double)part;
+                    if (g.mode == 0) nacc += row_weights<0>(c, f, f.y, q, g.K4, lane);
+                    else if (g.mode == 2) nacc += row_weights<2>(c, f, f.y, q, g.K4, lane);
+                    else nacc += row_weights<1>(c, f, q.x, q, g.K4, lane);
                 }
                 wave_sync();
             }
@@ -166,7 +185,7 @@ __global__ __launch_bounds__(256) void k_sky_norm(DiscRec *__restrict__ recs, in
 // The batch's particles on the current ring, and the LDS copy of one
 // segment of the ring.
 struct GatherLDS {
-    float4 f[64]; // A, B (fast form) or E (general form), offm, step
+    float4 f[64]; // A, B (series forms) or E (general form), offm, step
     float4 q[64]; // Ac, D, K4, flux / norm
     int4 c[64];   // first column in [0, nr), count, kc_rel, weight form
     unsigned cand[64];
@@ -182,6 +201,41 @@ __device__ __forceinline__ int gather_ring(int k, int nside)
     return (j & 1) == 0 ? nside - 1 - (j >> 1) : 3 * nside + 1 + (j >> 1);
 }
 
+// Adds one particle's columns in a ring segment into the LDS copy: while
+// the wave has 256 columns left, four per lane at a time -- their weights
+// first, then the four slot reads, then the four writes (the columns are
+// distinct, so the slots are), four independent read-modify-writes in
+// flight -- then one at a time.
+template <int MODE>
+__device__ __forceinline__ void gather_row(float *win, const SegSpan &sp, const int4 &ci, const float4 &f,
+                                           const float4 &q, int nr, int s0, int lane)
+{
+    const int total = sp.na + sp.nb;
+    int i0 = 0;
+    for (; i0 + 256 <= total; i0 += 256) {
+        int sl[4];
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = i0 + 64 * k + lane;
+            const int t = i < sp.na ? sp.u + i : i - sp.na;
+            v[k] = weight32<MODE>(fmaf((float)(t - ci.z), f.w, f.z), f.x, f.y, q.x, q.y, q.z) * q.w;
+            sl[k] = seg_slot(ci.x, t, nr, s0);
+        }
+        float a[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] = win[sl[k]];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) win[sl[k]] = a[k] + v[k];
+    }
+    for (int i = i0 + lane; i < total; i += 64) {
+        const int t = i < sp.na ? sp.u + i : i - sp.na;
+        const float v = weight32<MODE>(fmaf((float)(t - ci.z), f.w, f.z), f.x, f.y, q.x, q.y, q.z) * q.w;
+        const int sl = seg_slot(ci.x, t, nr, s0);
+        win[sl] = win[sl] + v;
+    }
+}
+
 // Renders the batch S.cand[0..cn) on ring r (first pixel startpix, nr
 // pixels).
 template <int ACC>
@@ -194,7 +248,7 @@ __device__ void gather_batch(GatherLDS &S, int cn, const DiscRec *__restrict__ r
         const RingRec rr = ring_setup(g, nside, ring_index(g, r));
         int c0 = rr.col0 % nr;
         if (c0 < 0) c0 += nr;
-        S.f[lane] = make_float4(rr.A, g.mode == 0 ? rr.B : rr.E, rr.offm, rr.step);
+        S.f[lane] = make_float4(rr.A, g.mode == 1 ? rr.E : rr.B, rr.offm, rr.step);
         S.q[lane] = make_float4(rr.Ac, rr.D, g.K4, (float)rec.flux);
         S.c[lane] = make_int4(c0, rr.cnt, rr.kc_rel, g.mode);
     }
@@ -208,15 +262,15 @@ __device__ void gather_batch(GatherLDS &S, int cn, const DiscRec *__restrict__ r
             if (sp.na + sp.nb == 0) continue;
             const float4 f = S.f[c];
             const float4 q = S.q[c];
-            for (int i = lane; i < sp.na + sp.nb; i += 64) {
-                const int t = i < sp.na ? sp.u + i : i - sp.na;
-                const float x = fmaf((float)(t - ci.z), f.w, f.z);
-                const float w = ci.w == 0 ? weight32<0>(x, f.x, f.y, 0.0f, 0.0f, q.z)
-                                          : weight32<1>(x, f.x, f.y, q.x, q.y, q.z);
-                S.win[seg_slot(ci.x, t, nr, s0)] += w * q.w;
-            }
-            wave_sync();
+            if (ci.w == 0) gather_row<0>(S.win, sp, ci, f, q, nr, s0, lane);
+            else if (ci.w == 2) gather_row<2>(S.win, sp, ci, f, q, nr, s0, lane);
+            else gather_row<1>(S.win, sp, ci, f, q, nr, s0, lane);
+            // the next particle's slot reads are ordered after these
+            // writes by the wave's in-order LDS queue; only the compiler
+            // must not move them
+            asm volatile("" ::: "memory");
         }
+        wave_sync();
         // the segment into the map; its slots go back to zero
         for (int s = lane; s < ws; s += 64) {
             const float v = S.win[s];
@@ -229,6 +283,8 @@ __device__ void gather_batch(GatherLDS &S, int cn, const DiscRec *__restrict__ r
                     const float err = (old - (sum - bb)) + (v - bb);
                     map.f32[p] = sum;
                     if (err != 0.0f) map.lo[p] += err;
+                } else if constexpr (ACC == kAccFix) {
+                    map.fix[p] += (unsigned long long)__double2ll_rn((double)v * map.fix_scale);
                 } else {
                     map.f32[p] += v;
                 }
@@ -260,22 +316,33 @@ __global__ __launch_bounds__(64) void k_sky_gather(const DiscRec *__restrict__ r
         int startpix, nr, shifted;
         ring_info(nside, r, startpix, nr, shifted);
         int cn = 0;
-        for (unsigned base = 0; base < nsky; base += 64) {
-            const unsigned i = base + (unsigned)lane;
-            bool hit = false;
-            if (i < nsky) {
-                const unsigned pr = ranges[i];
-                hit = (int)(pr & 0xffffu) <= r && r <= (int)(pr >> 16);
+        // the scan reads four packed ranges per lane at a time (one 16-byte
+        // load; the buffer and every 256-entry block start are 16-byte
+        // aligned), entries past nsky read as the empty range
+        for (unsigned base = 0; base < nsky; base += 256) {
+            const unsigned i0 = base + 4u * (unsigned)lane;
+            uint4 pr4 = make_uint4(1u, 1u, 1u, 1u);
+            if (i0 + 3u < nsky) {
+                pr4 = *reinterpret_cast<const uint4 *>(ranges + i0);
+            } else {
+                if (i0 < nsky) pr4.x = ranges[i0];
+                if (i0 + 1u < nsky) pr4.y = ranges[i0 + 1u];
+                if (i0 + 2u < nsky) pr4.z = ranges[i0 + 2u];
             }
-            const uint64_t hm = __ballot(hit);
-            const int nh = __popcll(hm);
-            if (cn + nh > 64) {
-                gather_batch<ACC>(S, cn, recs, n, nside, pad, r, startpix, nr, map, lane);
-                cn = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned pr = k == 0 ? pr4.x : k == 1 ? pr4.y : k == 2 ? pr4.z : pr4.w;
+                const bool hit = (int)(pr & 0xffffu) <= r && r <= (int)(pr >> 16);
+                const uint64_t hm = __ballot(hit);
+                const int nh = __popcll(hm);
+                if (cn + nh > 64) {
+                    gather_batch<ACC>(S, cn, recs, n, nside, pad, r, startpix, nr, map, lane);
+                    cn = 0;
+                }
+                if (hit) S.cand[cn + __popcll(hm & ((1ull << lane) - 1ull))] = i0 + (unsigned)k;
+                cn += nh;
+                wave_sync();
             }
-            if (hit) S.cand[cn + __popcll(hm & ((1ull << lane) - 1ull))] = i;
-            cn += nh;
-            wave_sync();
         }
         if (cn > 0) gather_batch<ACC>(S, cn, recs, n, nside, pad, r, startpix, nr, map, lane);
     }
@@ -292,6 +359,9 @@ hipError_t launch_sky(int acc, DiscRec *recs, int64_t n, unsigned *counts, int n
     } else if (acc == kAccF32) {
         k_sky_norm<kAccF32><<<norm_blocks, 256, 0, st>>>(recs, n, counts, nside, pad, map, stats, ranges);
         k_sky_gather<kAccF32><<<gather_blocks, 64, 0, st>>>(recs, n, counts, nside, pad, map, ranges);
+    } else if (acc == kAccFix) {
+        k_sky_norm<kAccFix><<<norm_blocks, 256, 0, st>>>(recs, n, counts, nside, pad, map, stats, ranges);
+        k_sky_gather<kAccFix><<<gather_blocks, 64, 0, st>>>(recs, n, counts, nside, pad, map, ranges);
     } else {
         return hipErrorInvalidValue;
     }

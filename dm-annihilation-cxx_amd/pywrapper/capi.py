@@ -17,8 +17,8 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libdmann.so")
 
-PREC_FP32, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2
-PRECISIONS = {"fp32": PREC_FP32, "compensated": PREC_COMPENSATED, "fp64": PREC_FP64}
+PREC_FP32, PREC_COMPENSATED, PREC_FP64, PREC_FIXED = 0, 1, 2, 3
+PRECISIONS = {"fp32": PREC_FP32, "compensated": PREC_COMPENSATED, "fp64": PREC_FP64, "fixed": PREC_FIXED}
 
 # 48-byte DMParticle record (src/datatypes.h:46-105)
 PARTICLE_FIELDS = ("mass", "dens", "hsmooth", "posx", "posy", "posz",
@@ -299,7 +299,7 @@ class Projector:
         _check(self._L.dmann_set_tuning(self._h, int(medium_blocks_per_cu), int(large_blocks_per_cu),
                                         int(large_pix)), self._h)
 
-    def set_sky_gather(self, blocks_per_cu: int = 8):
+    def set_sky_gather(self, blocks_per_cu: int = 12):
         """Sky-covering class as a per-ring gather with that many
         single-wavefront workgroups per CU (0: one workgroup per particle
         with per-pixel atomics)."""

@@ -43,11 +43,26 @@ typedef struct dmann_particle {
     float sigmav, eps, phi;
 } dmann_particle;
 
-/* Accumulation / arithmetic form of a context. */
+/* Accumulation / arithmetic form of a context.
+ *
+ * DMANN_PREC_FIXED (no counterpart in the reference) gives the same map,
+ * bit for bit, every time the same particles are rendered in the same calls:
+ * the pre-pass places records in particle order (per-tile counts and their
+ * prefix sums instead of atomic appends), the grouped kernel's neighbours
+ * come from a stable radix sort, and every deposit -- FP32 weight times
+ * flux / norm, or a window's FP32 partial sum -- is rounded to a 64-bit
+ * fixed-point map whose integer additions commute.  The map's scale 2^E is
+ * set at the first render after a reset so that the kept flux so far sits
+ * below 2^58 units (every pixel is at most that flux), read back from the
+ * device, so each render call synchronises once; a later render that would
+ * pass 2^61 shifts the map down first.  Accuracy is that of the compensated
+ * form (FP32 weights and window partial sums, exact accumulation).  Single
+ * maps only (not dmann_batch_*, not dmann_render_map_noalloc). */
 enum dmann_precision {
     DMANN_PREC_FP32 = 0,        /* FP32 weights, plain FP32 atomic accumulation          */
     DMANN_PREC_COMPENSATED = 1, /* FP32 weights, two-float (hi+lo) compensated accumulation */
-    DMANN_PREC_FP64 = 2         /* FP64 weights as the reference computes them; FP64 map  */
+    DMANN_PREC_FP64 = 2,        /* FP64 weights as the reference computes them; FP64 map  */
+    DMANN_PREC_FIXED = 3        /* FP32 weights, 64-bit fixed-point map: bit-reproducible */
 };
 
 /* Status codes. */
@@ -184,7 +199,7 @@ int dmann_set_subpixel_sort(dmann_ctx *ctx, int on);
  * wavefront per HEALPix ring owning that ring's pixels, accumulating up to
  * 64 particles at a time in LDS and adding them into the map without
  * atomics -- with that many single-wavefront workgroups per CU; 0 renders
- * it with one workgroup per particle and per-pixel atomics.  Default 8.
+ * it with one workgroup per particle and per-pixel atomics.  Default 12.
  * The gather keeps each particle's ring range in the grouping's key
  * buffer: with grouping off (dmann_set_grouping K = 0), and in the FP64
  * form, the per-particle kernel is used. */
