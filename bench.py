@@ -88,10 +88,16 @@ def time_form(pw, torch, parts, nside, precision, steps, warmup):
         pr.timing(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for k in range(steps):
             pr.reset()
             pr.render(parts)
             pr.get_map_device(out)
+            if k % 5 == 4 and k + 1 < steps:
+                # a progress line: wait for the queued steps first (the
+                # timed region stays one stream of renders; the wait adds
+                # no GPU work)
+                torch.cuda.synchronize()
+                log(f"{precision}: {k + 1} of {steps} timed steps, {time.perf_counter() - t0:.1f} s")
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         kms, calls = pr.timing(False)

@@ -168,53 +168,61 @@ __device__ __forceinline__ float seg_mask(float lenf, int t)
     return __builtin_amdgcn_fmed3f(lenf - (float)t, 0.0f, 1.0f);
 }
 
-// Walks the table's segments round-robin: in iteration i lane L takes
-// segment 64 i + L, so the 64 lanes hold consecutive segments -- within a
-// ring their window slots are kSeg (odd) apart, in distinct LDS banks, and
-// their ring-term reads mostly hit one table row (a broadcast).  kSeg
-// pixels per segment with the ring terms loaded once per segment, the
-// segment's table ring from the segment-ring table (beyond it, found by
-// stepping from the last tabled ring).  Every lane evaluates all kSeg pixels of its segment; those past
-// the ring's end are masked to zero (a factor 0 in an fma: the weights
-// there are finite), so the loop has no divergent branch.  Pixel 0 of a
-// segment always lies on the ring.
-// MODE 0 sums the weights (returns the lane's sum).  MODE 1 deposits
-// weight * sc: into the LDS window (WINDOW), where each band ring owns
-// window-columns + kSeg slots, so the masked tail of a ring's last segment
-// lands in slots no other lane touches; a segment that would leave its
-// ring's slots (which the window construction rules out) goes to global
-// atomics instead.  Without WINDOW every deposit is a global atomic.
-template <int MODE, int ACC, bool WINDOW, int PV>
-__device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, float K4, float sc, const MapRef &map)
+// Table ring of segment s: from the segment-ring table, or beyond it by
+// stepping from the last tabled ring (at least q_from).
+__device__ __forceinline__ int seg_ring(const GroupLDS &S, int s, int q_from)
+{
+    if (s < kSegTab) return S.q0[s];
+    int q = q_from < S.q0[kSegTab - 1] ? S.q0[kSegTab - 1] : q_from;
+    while (S.ri[q].w <= s) ++q;
+    return q;
+}
+
+// Deposits weight * sc for the table's segments, walked round-robin: in
+// iteration i lane L takes segment 64 i + L, so the 64 lanes hold
+// consecutive segments -- within a ring their window slots are kSeg (odd)
+// apart, in distinct LDS banks, and their ring-term reads mostly hit one
+// table row (a broadcast).  kSeg pixels per segment with the ring terms
+// loaded once per segment.  Every lane evaluates all kSeg pixels of its
+// segment; those past the ring's end are masked to zero (a factor 0 in an
+// fma: the weights there are finite), so the loop has no divergent branch.
+// Pixel 0 of a segment always lies on the ring.
+// The LDS reads are software-pipelined: the table ring of segment s + 128
+// and the ring terms of segment s + 64 are read in the iteration of
+// segment s, before its window slots, so each iteration waits for one LDS
+// round trip instead of three (ring, terms, slots).  The table is read-only
+// during the walk and the slots are a separate array, so the reordering
+// changes no value.
+// WINDOW: into the LDS window, where each band ring owns window-columns +
+// kSeg slots, so the masked tail of a ring's last segment lands in slots no
+// other lane touches; a segment that would leave its ring's slots (which
+// the window construction rules out) goes to global atomics instead.
+// Without WINDOW every deposit is a global atomic.
+template <int ACC, bool WINDOW, int PV>
+__device__ __forceinline__ void deposit_segments(GroupLDS &S, int NS, int lane, float K4, float sc,
+                                                 const MapRef &map)
 {
     const int nmax = (NS + 63) >> 6;
-    double total = 0.0;
-    float part = 0.0f;
-    int q = 0;
+    int s = lane;
+    int q = s < NS ? seg_ring(S, s, 0) : 0;
+    int qn = s + 64 < NS ? seg_ring(S, s + 64, q) : q;
+    float4 f = S.rf[q];
+    int4 r = S.ri[q];
+    float rx = S.rx[q];
+    int4 w = S.rs[q];
     for (int i = 0; i < nmax; ++i) {
-        const int s = 64 * i + lane;
+        // next segment's terms and the ring of the one after (read-only)
+        const int qn2 = s + 128 < NS ? seg_ring(S, s + 128, qn) : qn;
+        const float4 fn = S.rf[qn];
+        const int4 rn = S.ri[qn];
+        const float rxn = S.rx[qn];
+        const int4 wn = S.rs[qn];
         if (s < NS) {
-            if (s < kSegTab) {
-                q = S.q0[s];
-            } else {
-                if (q < S.q0[kSegTab - 1]) q = S.q0[kSegTab - 1];
-                while (S.ri[q].w <= s) ++q;
-            }
-            const float4 f = S.rf[q];
-            const int4 r = S.ri[q];
             const int t0 = (s - r.z) * kSeg;
             const int len = r.y - t0 < kSeg ? r.y - t0 : kSeg;
             const float lenf = (float)len;
-            const float x0 = fmaf((float)(t0 - r.x), f.w, S.rx[q]);
-            if constexpr (MODE == 0) {
-#pragma unroll
-                for (int t = 0; t < kSeg; ++t) {
-                    const float x = t == 0 ? x0 : fmaf((float)t, f.w, x0);
-                    const float w = ring_term_weight<PV>(x, f.x, f.y, f.z, K4);
-                    part = t == 0 ? part + w : fmaf(w, seg_mask(lenf, t), part);
-                }
-            } else if constexpr (WINDOW) {
-                const int4 w = S.rs[q];
+            const float x0 = fmaf((float)(t0 - r.x), f.w, rx);
+            if constexpr (WINDOW) {
                 int off = w.z + t0;
                 if (off >= w.w) off -= w.w;
                 if (off + len <= w.y) {
@@ -232,38 +240,47 @@ __device__ __forceinline__ double walk_segments(GroupLDS &S, int NS, int lane, f
                 deposit_direct<ACC, PV>(S, q, t0, len, x0, f, K4, sc, map);
             }
         }
-        if ((i & 3) == 3) {
-            total += (double)part;
-            part = 0.0f;
-        }
+        s += 64;
+        q = qn;
+        qn = qn2;
+        f = fn;
+        r = rn;
+        rx = rxn;
+        w = wn;
     }
-    return total + (double)part;
 }
 
 // Sum of the weights of one table row held in a lane's registers: pixels
 // t = 0 .. cnt-1 at longitude offset x_t = offm + (t - kc_rel) step, four at
 // a time from x of the first (two roundings, as in the segment walks), the
-// last 0-3 masked.  FP32 partial sums of at most 16 weights go to the FP64
-// total.  The trip count is the lane's own: rows of different lengths
-// share the wavefront.
+// last 0-3 masked.  FP32 partial sums of 16 weights (four groups of four)
+// go to the FP64 total once per 16.  The trip count is the lane's own:
+// rows of different lengths share the wavefront.
+template <int PV>
+__device__ __forceinline__ float four_weights(const RingLite &rl, int t0, float K4)
+{
+    const float xb = fmaf((float)(t0 - rl.kc_rel), rl.step, rl.offm);
+    const float w0 = ring_term_weight<PV>(xb, rl.e0, rl.e1, rl.e2, K4);
+    const float w1 = ring_term_weight<PV>(fmaf(1.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
+    const float w2 = ring_term_weight<PV>(fmaf(2.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
+    const float w3 = ring_term_weight<PV>(fmaf(3.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
+    return (w0 + w1) + (w2 + w3);
+}
+
 template <int PV>
 __device__ __forceinline__ double row_sum(const RingLite &rl, float K4)
 {
     const int n4 = rl.cnt >> 2;
     double acc = 0.0;
-    float part = 0.0f;
-    for (int c = 0; c < n4; ++c) {
-        const float xb = fmaf((float)(4 * c - rl.kc_rel), rl.step, rl.offm);
-        const float w0 = ring_term_weight<PV>(xb, rl.e0, rl.e1, rl.e2, K4);
-        const float w1 = ring_term_weight<PV>(fmaf(1.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
-        const float w2 = ring_term_weight<PV>(fmaf(2.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
-        const float w3 = ring_term_weight<PV>(fmaf(3.0f, rl.step, xb), rl.e0, rl.e1, rl.e2, K4);
-        part += (w0 + w1) + (w2 + w3);
-        if ((c & 3) == 3) {
-            acc += (double)part;
-            part = 0.0f;
-        }
+    int c = 0;
+    for (; c + 4 <= n4; c += 4) {
+        float part = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part += four_weights<PV>(rl, 4 * (c + k), K4);
+        acc += (double)part;
     }
+    float part = 0.0f;
+    for (; c < n4; ++c) part += four_weights<PV>(rl, 4 * c, K4);
     const int t0 = 4 * n4;
     const float lenf = (float)(rl.cnt - t0);
     const float xb = fmaf((float)(t0 - rl.kc_rel), rl.step, rl.offm);
@@ -275,14 +292,15 @@ __device__ __forceinline__ double row_sum(const RingLite &rl, float K4)
     return acc + (double)part;
 }
 
-// walk_segments with the particle's series lengths (fast_variant).
-template <int MODE, int ACC, bool WINDOW>
-__device__ __forceinline__ double walk_pv(int pv, GroupLDS &S, int NS, int lane, float K4, float sc, const MapRef &map)
+// deposit_segments with the particle's series lengths (fast_variant).
+template <int ACC, bool WINDOW>
+__device__ __forceinline__ void deposit_pv(int pv, GroupLDS &S, int NS, int lane, float K4, float sc,
+                                           const MapRef &map)
 {
     switch (pv) {
-    case 0: return walk_segments<MODE, ACC, WINDOW, 0>(S, NS, lane, K4, sc, map);
-    case 1: return walk_segments<MODE, ACC, WINDOW, 1>(S, NS, lane, K4, sc, map);
-    default: return walk_segments<MODE, ACC, WINDOW, 2>(S, NS, lane, K4, sc, map);
+    case 0: deposit_segments<ACC, WINDOW, 0>(S, NS, lane, K4, sc, map); break;
+    case 1: deposit_segments<ACC, WINDOW, 1>(S, NS, lane, K4, sc, map); break;
+    default: deposit_segments<ACC, WINDOW, 2>(S, NS, lane, K4, sc, map); break;
     }
 }
 
@@ -621,8 +639,8 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
                 const float sc = wave_read(my_scale, j);
                 const float K4 = wave_read(my_K4, j);
                 const int pv = wave_read(my_pv, j);
-                if (direct) walk_pv<1, ACC, false>(pv, S, NS, lane, K4, sc, map);
-                else walk_pv<1, ACC, true>(pv, S, NS, lane, K4, sc, map);
+                if (direct) deposit_pv<ACC, false>(pv, S, NS, lane, K4, sc, map);
+                else deposit_pv<ACC, true>(pv, S, NS, lane, K4, sc, map);
                 wave_sync();
                 j = jn;
             }
