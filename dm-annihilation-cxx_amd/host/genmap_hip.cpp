@@ -17,9 +17,10 @@
 // runs on the device.  The map stays on the device across chunks and comes
 // back once, already divided by dOmega.
 //
-// Accumulation form: DMANN_PRECISION=fp32 | compensated | fp64 in the
-// environment (default compensated, which meets the 1e-5 per-pixel bound
-// against the FP64 CPU generator).  Flux plug-in: DMANN_FLUX=ann_nosom
+// Accumulation form: DMANN_PRECISION=fp32 | compensated | fp64 | fixed in
+// the environment (default compensated, which meets the 1e-5 per-pixel
+// bound against the FP64 CPU generator; fixed gives the same map bit for
+// bit on every run).  Flux plug-in: DMANN_FLUX=ann_nosom
 // (default) | ann_pwave | ann_somv | ann_somv2 | decay | ann_adiabatic |
 // ann_core_nosom (cores from ./halocore/VL2_Halos.data or DMANN_HALO_FILE) -- the
 // reference links one src/flux/flux_<kind>.cpp per executable
@@ -44,6 +45,7 @@ static int precision_from_env()
     if (!s || !*s) return DMANN_PREC_COMPENSATED;
     if (!strcmp(s, "fp32")) return DMANN_PREC_FP32;
     if (!strcmp(s, "fp64")) return DMANN_PREC_FP64;
+    if (!strcmp(s, "fixed")) return DMANN_PREC_FIXED;
     return DMANN_PREC_COMPENSATED;
 }
 
