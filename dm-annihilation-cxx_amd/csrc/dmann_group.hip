@@ -304,6 +304,20 @@ __device__ __forceinline__ void deposit_pv(int pv, GroupLDS &S, int NS, int lane
     }
 }
 
+// The lo words of FB two-float additions whose hi atomics returned old:
+// the exact rounding error of old + v (TwoSum).
+template <int FB>
+__device__ __forceinline__ void two_float_lo(const MapRef &map, const float *old, const float *v, const int *pix)
+{
+#pragma unroll
+    for (int u = 0; u < FB; ++u) {
+        const float sum = old[u] + v[u];
+        const float bb = sum - old[u];
+        const float err = (old[u] - (sum - bb)) + (v[u] - bb);
+        if (err != 0.0f) unsafeAtomicAdd(map.lo + pix[u], err);
+    }
+}
+
 // Flushes the band's window slots 0 .. Sb-1 (Sb >= 1): one global atomic
 // (form) per non-zero slot.  Slots are taken FB x 64 at a time with
 // unconditional (clamped) LDS reads, so the lookups of the FB chunks overlap
@@ -313,6 +327,9 @@ template <int ACC>
 __device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const MapRef &map, unsigned npix)
 {
     constexpr int FB = ACC == kAccComp ? 8 : 4;
+    float pold[FB], pv[FB];
+    int ppix[FB];
+    bool have = false;
     for (int t0 = 0; t0 < Sb; t0 += 64 * FB) {
         int tc[FB], j[FB], pix[FB];
         float v[FB];
@@ -336,26 +353,31 @@ __device__ __forceinline__ void flush_band(GroupLDS &S, int Sb, int lane, const 
         }
         if constexpr (ACC == kAccComp) {
             // two-float accumulation: hi by a returning atomic, then the
-            // exact rounding error of that addition (TwoSum) into lo
+            // exact rounding error of that addition (TwoSum) into lo.  The
+            // lo atomics of a chunk are issued after the next chunk's hi
+            // atomics, so two chunks of returning atomics are in flight
             float old[FB];
 #pragma unroll
             for (int u = 0; u < FB; ++u) {
                 old[u] = 0.0f;
                 if (v[u] != 0.0f) old[u] = unsafeAtomicAdd(map.f32 + pix[u], v[u]);
             }
+            if (have) two_float_lo<FB>(map, pold, pv, ppix);
 #pragma unroll
             for (int u = 0; u < FB; ++u) {
-                const float sum = old[u] + v[u];
-                const float bb = sum - old[u];
-                const float err = (old[u] - (sum - bb)) + (v[u] - bb);
-                if (err != 0.0f) unsafeAtomicAdd(map.lo + pix[u], err);
+                pold[u] = old[u];
+                pv[u] = v[u];
+                ppix[u] = pix[u];
             }
+            have = true;
         } else {
 #pragma unroll
             for (int u = 0; u < FB; ++u)
                 if (v[u] != 0.0f) acc_add<ACC>(map, pix[u], v[u], 0.0);
         }
     }
+    if constexpr (ACC == kAccComp)
+        if (have) two_float_lo<FB>(map, pold, pv, ppix);
 }
 
 // A band of window rings b0 .. b0+BH-1 (BH <= 64, rings up to R1) whose
