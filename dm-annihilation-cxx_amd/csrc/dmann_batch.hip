@@ -233,7 +233,7 @@ int dmann_batch_create(dmann_batch **out, const dmann_params *params, int nmaps,
     b->cfg.medium_blocks = b->cus * 8;
     b->cfg.large_blocks = b->cus * 4;
     b->cfg.sky_blocks = b->cus * 12;
-    b->gc.k = 32;
+    b->gc.k = kDefaultGroupK;
     b->gc.sort_blocks = b->cus * 16;
     b->gc.group_blocks = b->cus * DMANN_GROUP_BPC;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);

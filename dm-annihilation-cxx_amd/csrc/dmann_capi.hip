@@ -98,12 +98,16 @@ static void free_group(dmann_ctx *c)
 {
     (void)hipFree(c->gc.keys);
     (void)hipFree(c->gc.idx);
-    (void)hipFree(c->gc.hist);
-    (void)hipFree(c->gc.sums);
+    (void)hipFree(c->gc.keys_out);
+    (void)hipFree(c->gc.iota);
+    (void)hipFree(c->gc.sort_tmp);
     (void)hipFree(c->gc.work);
     (void)hipFree(c->gc.tab_f);
     (void)hipFree(c->gc.tab_i);
-    c->gc.keys = c->gc.idx = c->gc.hist = c->gc.sums = c->gc.work = nullptr;
+    c->gc.keys = c->gc.idx = c->gc.keys_out = c->gc.iota = c->gc.work = nullptr;
+    c->gc.sort_tmp = nullptr;
+    c->gc.sort_tmp_bytes = 0;
+    c->gc.hist = c->gc.sums = nullptr;
     c->gc.tab_f = nullptr;
     c->gc.tab_i = nullptr;
 }
@@ -119,19 +123,22 @@ static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
     c->gc.nside_c = nside_c;
     if (k <= 0 || c->acc == kAccF64) return hipSuccess;
     const int64_t cap = c->rec_cap ? c->rec_cap : 1;
-    const int64_t nbins = 12LL * nside_c * nside_c * kSizeBuckets;
     const size_t rows = (size_t)c->gc.group_blocks * (size_t)k * 64;
+    const size_t bytes = group_sort_bytes(cap);
+    if (bytes == 0) return hipErrorInvalidValue;
     hipError_t e;
     if ((e = hipMalloc(&c->gc.keys, cap * sizeof(unsigned))) != hipSuccess ||
         (e = hipMalloc(&c->gc.idx, cap * sizeof(unsigned))) != hipSuccess ||
-        (e = hipMalloc(&c->gc.hist, nbins * sizeof(unsigned))) != hipSuccess ||
-        (e = hipMalloc(&c->gc.sums, ((nbins + 1023) / 1024) * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.keys_out, cap * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.iota, cap * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&c->gc.sort_tmp, bytes)) != hipSuccess ||
         (e = hipMalloc(&c->gc.work, sizeof(unsigned))) != hipSuccess ||
         (e = hipMalloc(&c->gc.tab_f, rows * sizeof(float4))) != hipSuccess ||
         (e = hipMalloc(&c->gc.tab_i, rows * sizeof(int4))) != hipSuccess) {
         free_group(c);
         return e;
     }
+    c->gc.sort_tmp_bytes = bytes;
     c->gc.k = k;
     return hipSuccess;
 }
@@ -139,7 +146,7 @@ static hipError_t alloc_group(dmann_ctx *c, int k, int nside_c)
 // True when grouped rendering is configured with all of its buffers.
 static bool group_ready(const GroupCfg &gc)
 {
-    return gc.k > 0 && gc.keys && gc.idx && gc.hist && gc.sums && gc.work && gc.tab_f && gc.tab_i;
+    return gc.k > 0 && gc.keys && gc.idx && gc.keys_out && gc.iota && gc.sort_tmp && gc.work && gc.tab_f && gc.tab_i;
 }
 
 static void free_halos(dmann_ctx *c)
@@ -179,9 +186,6 @@ static void free_det(dmann_ctx *c)
     (void)hipFree(c->det.tile_sums);
     (void)hipFree(c->det.flux_part);
     (void)hipFree(c->det.flux_total);
-    (void)hipFree(c->det.keys_out);
-    (void)hipFree(c->det.iota);
-    (void)hipFree(c->det.sort_tmp);
     c->det = DetCfg{};
 }
 
@@ -191,22 +195,16 @@ static hipError_t alloc_det(dmann_ctx *c)
     free_det(c);
     const int64_t cap = c->rec_cap ? c->rec_cap : 1;
     const int64_t tiles = (cap + 511) / 512 + 1;
-    const size_t bytes = det_sort_bytes(cap);
-    if (bytes == 0) return hipErrorInvalidValue;
     hipError_t e;
     if ((e = hipMalloc(&c->det.tile_med, tiles * sizeof(unsigned))) != hipSuccess ||
         (e = hipMalloc(&c->det.tile_large, tiles * sizeof(unsigned))) != hipSuccess ||
         (e = hipMalloc(&c->det.tile_sums, ((tiles + 1023) / 1024) * sizeof(unsigned))) != hipSuccess ||
         (e = hipMalloc(&c->det.flux_part, (c->cfg.prepass_blocks > 0 ? c->cfg.prepass_blocks : 1) * sizeof(double))) !=
             hipSuccess ||
-        (e = hipMalloc(&c->det.flux_total, sizeof(double))) != hipSuccess ||
-        (e = hipMalloc(&c->det.keys_out, cap * sizeof(unsigned))) != hipSuccess ||
-        (e = hipMalloc(&c->det.iota, cap * sizeof(unsigned))) != hipSuccess ||
-        (e = hipMalloc(&c->det.sort_tmp, bytes)) != hipSuccess) {
+        (e = hipMalloc(&c->det.flux_total, sizeof(double))) != hipSuccess) {
         free_det(c);
         return e;
     }
-    c->det.sort_tmp_bytes = bytes;
     return hipSuccess;
 }
 
@@ -435,7 +433,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
         return bail("hipMalloc stats", e);
     c->gc.sort_blocks = dev_cus * 16;
     c->gc.group_blocks = dev_cus * DMANN_GROUP_BPC;
-    if ((e = alloc_group(c, 32, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
+    if ((e = alloc_group(c, kDefaultGroupK, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
     if (c->acc == kAccFix && (e = alloc_det(c)) != hipSuccess) return bail("hipMalloc reproducible-form buffers", e);
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);

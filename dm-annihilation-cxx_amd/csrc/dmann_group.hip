@@ -609,8 +609,9 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
             const bool direct = bd.direct;
             const int b1 = b0 + BH - 1;
             // members meeting the band, in order (a ballot of them, taken
-            // lowest first); the cached ring terms of the next one are
-            // loaded while the current one is walked
+            // lowest first); the cached ring terms of the next two are
+            // loaded while the current one is walked (the cache rows come
+            // from L2 or memory)
             uint64_t meet = __ballot(my_on && my_hi >= b0 && my_lo <= b1);
             auto next_member = [&](int) {
                 if (meet == 0) return nm;
@@ -628,14 +629,18 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
                 }
             };
             int j = next_member(-1);
-            float4 nf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            int4 ni = make_int4(0, 0, 0, 0);
+            float4 nf = make_float4(0.0f, 0.0f, 0.0f, 0.0f), nf2 = nf;
+            int4 ni = make_int4(0, 0, 0, 0), ni2 = ni;
             if (j < nm) fetch(j, nf, ni);
+            int jn = next_member(j);
+            if (jn < nm) fetch(jn, nf2, ni2);
             while (j < nm) {
                 const float4 cf = nf;
                 const int4 ci = ni;
-                const int jn = next_member(j);
-                if (jn < nm) fetch(jn, nf, ni);
+                nf = nf2;
+                ni = ni2;
+                const int jn2 = next_member(jn);
+                if (jn2 < nm) fetch(jn2, nf2, ni2);
                 const int lo = wave_read(my_lo, j), hi = wave_read(my_hi, j);
                 const int rlo = b0 > lo ? b0 : lo, rhi = b1 < hi ? b1 : hi;
                 const bool on = lane <= rhi - rlo;
@@ -665,6 +670,7 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
                 else deposit_pv<ACC, true>(pv, S, NS, lane, K4, sc, map);
                 wave_sync();
                 j = jn;
+                jn = jn2;
             }
             if (!direct) {
                 flush_band<ACC>(S, Sb, lane, map, npix);
@@ -703,12 +709,11 @@ __global__ void k_sort_keys(const DiscRec *__restrict__ recs, const unsigned *__
     }
 }
 
-// Keys of the reproducible form's stable radix sort: records below the
-// medium count get their key, the rest of the nrec slots one past the
-// largest key (they sort last), and each slot its own index as the value,
-// so the sorted values list the medium records by key and, within a key,
-// in record order.
-__global__ void k_sort_keys_det(const DiscRec *__restrict__ recs, const unsigned *__restrict__ counts, int nside_c,
+// Keys of the stable radix sort: records below the medium count get their
+// key, the rest of the nrec slots one past the largest key (they sort
+// last), and each slot its own index as the value, so the sorted values
+// list the medium records by key and, within a key, in record order.
+__global__ void k_sort_keys_radix(const DiscRec *__restrict__ recs, const unsigned *__restrict__ counts, int nside_c,
                                 int64_t nrec, unsigned pad_key, unsigned *__restrict__ keys,
                                 unsigned *__restrict__ iota)
 {
@@ -790,20 +795,20 @@ __global__ void k_zero_u32(unsigned *p, int n)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-// Bits of the reproducible form's sort keys at nside_c: the largest key
-// is the pad key 12 nside_c^2 kSizeBuckets.
-static int det_key_bits(int nside_c)
+// Bits of the radix sort's keys at nside_c: the largest key is the pad key
+// 12 nside_c^2 kSizeBuckets.
+static int sort_key_bits(int nside_c)
 {
     const unsigned top = 12u * (unsigned)nside_c * (unsigned)nside_c * (unsigned)kSizeBuckets;
     return 32 - __builtin_clz(top);
 }
 
-size_t det_sort_bytes(int64_t n)
+size_t group_sort_bytes(int64_t n)
 {
     size_t bytes = 0;
     if (rocprim::radix_sort_pairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
                                   (const unsigned *)nullptr, (unsigned *)nullptr, (size_t)(n > 0 ? n : 1), 0,
-                                  det_key_bits(kMaxSortNside)) != hipSuccess)
+                                  sort_key_bits(kMaxSortNside)) != hipSuccess)
         return 0;
     return bytes;
 }
@@ -820,18 +825,21 @@ hipError_t launch_scan_u32(unsigned *data, int n, unsigned *sums, hipStream_t st
 
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, double pad, const GroupCfg &gc,
                         MapRef map,
-                        unsigned long long *stats, hipStream_t st, hipEvent_t sorted, const DetCfg *det, int64_t nrec)
+                        unsigned long long *stats, hipStream_t st, hipEvent_t sorted, int64_t nrec)
 {
     k_zero_u32<<<1, 64, 0, st>>>(gc.work, 1);
     const int nbins = 12 * gc.nside_c * gc.nside_c * kSizeBuckets;
-    if (det) {
+    if (gc.sort_tmp) {
         // stable radix sort of all nrec slots (the medium count is known on
-        // the device only): medium records by key, then by record order
-        k_sort_keys_det<<<gc.sort_blocks, 256, 0, st>>>(recs, counts, gc.nside_c, nrec, (unsigned)nbins, gc.keys,
-                                                          det->iota);
-        size_t bytes = det->sort_tmp_bytes;
-        const hipError_t e = rocprim::radix_sort_pairs(det->sort_tmp, bytes, gc.keys, det->keys_out, det->iota, gc.idx,
-                                                       (size_t)nrec, 0, det_key_bits(gc.nside_c), st);
+        // the device only): medium records by key, then by record order.
+        // At 1.1e9 records: 35 ms, against 100-150 ms for the counting sort,
+        // whose two returning atomics per record are bound by the L2's
+        // atomic rate.
+        k_sort_keys_radix<<<gc.sort_blocks, 256, 0, st>>>(recs, counts, gc.nside_c, nrec, (unsigned)nbins, gc.keys,
+                                                          gc.iota);
+        size_t bytes = gc.sort_tmp_bytes;
+        const hipError_t e = rocprim::radix_sort_pairs(gc.sort_tmp, bytes, gc.keys, gc.keys_out, gc.iota, gc.idx,
+                                                       (size_t)nrec, 0, sort_key_bits(gc.nside_c), st);
         if (e != hipSuccess) return e;
     } else {
         const int nb = (nbins + 1023) / 1024;

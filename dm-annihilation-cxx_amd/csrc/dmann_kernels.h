@@ -90,21 +90,18 @@ struct MapRef {
 
 // Buffers of the reproducible form (kAccFix): the pre-pass places records
 // by per-tile offsets (tiles of 512 particles in particle order) instead of
-// atomic appends, and the medium class is sorted by a stable radix sort.
+// atomic appends; the medium class is then sorted by the grouping's stable
+// radix sort, so groups are the same in every run.
 struct DetCfg {
     unsigned *tile_med;    // ceil(max_parts / 512): per-tile counts, then exclusive offsets
     unsigned *tile_large;
     unsigned *tile_sums;   // scan block sums, ceil(tiles / 1024)
     double *flux_part;     // per pre-pass block FP64 flux sums
     double *flux_total;    // their sum (one double)
-    unsigned *keys_out;    // sort output keys, max_parts
-    unsigned *iota;        // sort input values (record indices), max_parts
-    void *sort_tmp;        // radix sort temporary storage
-    size_t sort_tmp_bytes;
 };
 
 // Bytes of radix-sort temporary storage for n keys (dmann_group.hip).
-size_t det_sort_bytes(int64_t n);
+size_t group_sort_bytes(int64_t n);
 
 // Sum of the kept particles' fluxes (FP64, fixed reduction order) into
 // det.flux_total: a bound on every pixel of the map, which sets its
@@ -127,13 +124,21 @@ struct LaunchCfg {
     int sky_blocks;
 };
 
-// Grouped rendering of the medium class (dmann_group.hip): counting sort by
+// Grouped rendering of the medium class (dmann_group.hip): records sorted by
 // coarse sky cell, then K neighbours per wavefront with LDS accumulation.
 // Sort key of a medium record: coarse sky cell, then footprint-size class
 // (floor(log2 est) - 6, clamped to 0..15), so that a group of neighbours
 // also holds footprints of similar size and its window stays tight.
 constexpr int kSizeBuckets = 16;
 constexpr int kMaxSortNside = 1024;
+// Default group size.  The window's FP32 partial sums hold up to K deposits
+// per pixel, and deposits below half an ulp of the partial sum are lost, so
+// the compensated form's error grows with K: on 2e8 particles of the 1.1e9
+// workload at nside 2048, worst pixel / total against the FP64 map were
+// 1.2e-7 / 1.9e-9 at K = 16, 1.8e-7 / 5.6e-9 at K = 32 and 3.3e-7 / 1.6e-8
+// at K = 64, while K = 64 was only 2 % faster than K = 32 in the grouped
+// kernel (5033 against 5131 ms at 1.1e9 particles).
+constexpr int kDefaultGroupK = 32;
 
 struct GroupCfg {
     int k;            // particles per group (0: per-particle kernel instead)
@@ -142,8 +147,16 @@ struct GroupCfg {
     int group_blocks; // single-wave workgroups
     unsigned *keys;   // max_parts
     unsigned *idx;    // max_parts
-    unsigned *hist;   // 12 nside_c^2
-    unsigned *sums;   // ceil(12 nside_c^2 / 1024)
+    // radix sort (rocPRIM, stable): sorted keys and the record indices fed
+    // to it, max_parts each, and its temporary storage; without sort_tmp
+    // the counting sort below is used instead (its atomics are graph-safe,
+    // and the batch pipeline is captured in a hipGraph)
+    unsigned *keys_out;
+    unsigned *iota;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
+    unsigned *hist;   // counting sort: 12 nside_c^2 kSizeBuckets bins
+    unsigned *sums;   // ceil(bins / 1024)
     unsigned *work;   // group counter
     // per-wave cache of the members' ring terms (group_blocks x k x 64
     // rings), written by the normalisation pass, read back per band
@@ -154,7 +167,7 @@ struct GroupCfg {
 hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nside, double pad, const GroupCfg &gc,
                         MapRef map,
                         unsigned long long *stats, hipStream_t st,
-                        hipEvent_t sorted = nullptr, const DetCfg *det = nullptr, int64_t nrec = 0);
+                        hipEvent_t sorted = nullptr, int64_t nrec = 0);
 
 // Sky-covering class by normalisation per particle, then a gather per
 // ring (dmann_sky.hip); acc kAccF32, kAccComp or kAccFix; ranges: one word per

@@ -820,6 +820,8 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
     constexpr bool DET = ACC == kAccFix;
     constexpr bool SERIES = ACC == kAccF32 || ACC == kAccComp || ACC == kAccFix;
     if (DET && !det) return hipErrorInvalidValue;
+    // groups of the reproducible form come from the stable radix sort
+    if (DET && gc && gc->k > 0 && !gc->sort_tmp) return hipErrorInvalidValue;
     k_fill_u32<<<1, 64, 0, st>>>(counts, 4);
     const int64_t want = (n + 256 * kPrepassItems - 1) / (256 * kPrepassItems);
     const int pre_blocks = (int)(want < cfg.prepass_blocks ? want : cfg.prepass_blocks);
@@ -849,7 +851,7 @@ static hipError_t launch_render_t(const Particle *d_parts, int64_t n, const DevP
     if (ev) (void)hipEventRecord(ev[2], st);
     if (SERIES && gc && gc->k > 0) {
         const hipError_t e = launch_group(ACC, recs, counts, P.nside, P.footprint_pad, *gc, map, stats, st,
-                                          ev ? ev[3] : nullptr, DET ? det : nullptr, n);
+                                          ev ? ev[3] : nullptr, n);
         if (e != hipSuccess) return e;
     } else {
         if (ev) (void)hipEventRecord(ev[3], st);
