@@ -433,7 +433,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
         return bail("hipMalloc stats", e);
     c->gc.sort_blocks = dev_cus * 16;
     c->gc.group_blocks = dev_cus * DMANN_GROUP_BPC;
-    if ((e = alloc_group(c, kDefaultGroupK, p->nside >= 8 ? p->nside / 8 : 1)) != hipSuccess) return bail("hipMalloc sort cells", e);
+    if ((e = alloc_group(c, kDefaultGroupK, default_cell_nside(p->nside))) != hipSuccess) return bail("hipMalloc sort cells", e);
     if (c->acc == kAccFix && (e = alloc_det(c)) != hipSuccess) return bail("hipMalloc reproducible-form buffers", e);
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);
@@ -476,8 +476,8 @@ int dmann_set_grouping(dmann_ctx *c, int k, int nside_c)
 {
     if (!c) return DMANN_ERR_ARG;
     if (k < 0 || k > 64) return fail(c, DMANN_ERR_ARG, "group size %d outside 0..64", k);
-    if (nside_c < 1 || nside_c > c->P.nside || nside_c > kMaxSortNside)
-        return fail(c, DMANN_ERR_ARG, "sort cell nside %d outside 1..min(nside, %d)", nside_c, kMaxSortNside);
+    if (nside_c < 1 || nside_c > c->P.nside || nside_c > kMaxNestNside)
+        return fail(c, DMANN_ERR_ARG, "sort cell nside %d outside 1..min(nside, %d)", nside_c, kMaxNestNside);
     HIPCHK(c, hipSetDevice(c->device));
     FLUSH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -130,15 +130,23 @@ struct LaunchCfg {
 // (floor(log2 est) - 6, clamped to 0..15), so that a group of neighbours
 // also holds footprints of similar size and its window stays tight.
 constexpr int kSizeBuckets = 16;
-constexpr int kMaxSortNside = 1024;
+constexpr int kMaxSortNside = 1024; // counting sort (histogram of 12 nside_c^2 x 16 bins)
+constexpr int kMaxNestNside = 4096; // radix sort (class and NESTED cell in 32 key bits)
 // Default group size.  The window's FP32 partial sums hold up to K deposits
-// per pixel, and deposits below half an ulp of the partial sum are lost, so
-// the compensated form's error grows with K: on 2e8 particles of the 1.1e9
-// workload at nside 2048, worst pixel / total against the FP64 map were
-// 1.2e-7 / 1.9e-9 at K = 16, 1.8e-7 / 5.6e-9 at K = 32 and 3.3e-7 / 1.6e-8
-// at K = 64, while K = 64 was only 2 % faster than K = 32 in the grouped
-// kernel (5033 against 5131 ms at 1.1e9 particles).
-constexpr int kDefaultGroupK = 32;
+// per pixel; members deposit in ascending flux order, so small deposits are
+// summed before large ones (in arbitrary order a deposit below half an ulp
+// of the partial sum was lost).  On 2e8 particles of the 1.1e9 workload at
+// nside 2048, compensated, worst pixel / total against the FP64 map:
+// K = 32 1.0e-7 / 1.4e-10, K = 64 1.5e-7 / 1.1e-10 (in arbitrary order
+// K = 32 gave 1.8e-7 / 5.6e-9); the grouped kernel took 4743 ms at K = 64
+// against 4904 ms at K = 32 (1.1e9 particles).
+constexpr int kDefaultGroupK = 64;
+// Default coarse cell resolution of the radix sort: nside / 32 (cells of
+// 32 x 32 map pixels).  Grouped kernel, K = 64, coarse nside 2048/c at
+// 1.1e9 particles: c = 8 4925 ms, 16 4752, 32 4641, 64 4601, 128 4578,
+// 2048 (one cell per face) 4854; 1e7 particles at nside 1024: c = 8 233 ms,
+// 16 221, 32 210.
+inline int default_cell_nside(int nside) { return nside >= 32 ? nside / 32 : 1; }
 
 struct GroupCfg {
     int k;            // particles per group (0: per-particle kernel instead)

@@ -206,11 +206,12 @@ int dmann_set_subpixel_sort(dmann_ctx *ctx, int on);
 int dmann_set_sky_gather(dmann_ctx *ctx, int blocks_per_cu);
 
 /* Grouped rendering of the medium class (no counterpart in the reference):
- * particles are sorted by RING cell at nside_c (1..min(nside, 1024)), then
- * by footprint-size class, and K (1..64) neighbours share one wavefront and
- * an LDS window, cutting global atomics ~K-fold; k = 0 renders one particle
- * per wavefront with per-pixel atomics.  Defaults K = 32, nside_c =
- * nside / 8.  Ignored by the FP64 form. */
+ * particles are sorted by coarse NESTED cell at the largest power of two
+ * not above nside_c (1..min(nside, 4096)), then by footprint-size class,
+ * then by NESTED cell at the map's nside, and K (1..64) neighbours share
+ * one wavefront and an LDS window, cutting global atomics ~K-fold; k = 0
+ * renders one particle per wavefront with per-pixel atomics.  Defaults
+ * K = 64, nside_c = nside / 32.  Ignored by the FP64 form. */
 int dmann_set_grouping(dmann_ctx *ctx, int k, int nside_c);
 
 /* Uses `stream` (a hipStream_t, NULL = the null stream) for later calls. */

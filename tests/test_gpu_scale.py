@@ -34,7 +34,7 @@ def _render(pw, parts, nside, precision, k=None, chunks=1):
     return m, st
 
 
-@pytest.mark.parametrize("k", [16, 32])
+@pytest.mark.parametrize("k", [32, 64])
 def test_sample_5e6_nside2048_compensated_vs_fp64(pw, gpu, k):
     import torch
     print(f"case scale 5e6 nside 2048 k={k}", flush=True)
