@@ -55,6 +55,7 @@ def _render(pw, torch, parts, precision, chunks=1):
         pr.render(parts[: 1 << 20])  # warm-up (module load), then cleared
         pr.reset()
         torch.cuda.synchronize()
+        pr.timing(True)
         t0 = time.perf_counter()
         for i in range(0, n, cap):
             pr.render(parts[i:i + cap])
@@ -64,7 +65,9 @@ def _render(pw, torch, parts, precision, chunks=1):
         pr.get_map_device(out)
         torch.cuda.synchronize()
         sec = time.perf_counter() - t0
+        kms, _ = pr.timing(False)
         st = pr.stats()
+        st["kernel_ms"] = kms
     return out, sec, st, p
 
 
@@ -82,6 +85,9 @@ def test_headline_model_nside4096(pw, gpu):
     progress(f"precision at scale: plain {t_plain:.1f} s")
     ref, t_ref, st_ref, _ = _render(pw, torch, parts, "fp64", chunks=8)
     progress(f"precision at scale: fp64 {t_ref:.1f} s")
+    kms = st.pop("kernel_ms")
+    st_plain.pop("kernel_ms")
+    st_ref.pop("kernel_ms")
     for key in ("kept", "subpixel", "fallback", "extended", "contributions", "large"):
         assert st[key] == st_ref[key] == st_plain[key], key
     e_comp = _pixel_errors(comp, ref)
@@ -92,7 +98,8 @@ def test_headline_model_nside4096(pw, gpu):
     inv_ref = abs(float(ref.sum()) * d_omega - total) / total
     record("1.1e9 nside4096 compensated vs FP64", max_rel=e_comp["max_rel"], total_rel=e_comp["total_rel"],
            pixels=e_comp["pixels_checked"], flux_invariant=inv_comp, seconds=t_comp,
-           over_plain_time=t_comp / t_plain, contributions=st["contributions"])
+           over_plain_time=t_comp / t_plain, contributions=st["contributions"],
+           sky_covering_ms=kms["large"], grouped_ms=kms["medium"], sort_ms=kms["sort"], prepass_ms=kms["prepass"])
     record("1.1e9 nside4096 plain FP32 vs FP64", max_rel=e_plain["max_rel"], total_rel=e_plain["total_rel"],
            seconds=t_plain)
     record("1.1e9 nside4096 FP64", flux_invariant=inv_ref, seconds=t_ref)
