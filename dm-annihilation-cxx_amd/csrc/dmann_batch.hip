@@ -117,11 +117,11 @@ static void batch_free(dmann_batch *b)
     b->out.clear();
 }
 
-// Sort-cell resolution of a map in the per-map pipeline (the single-map
-// default: nside / 8).
+// Coarse sort-cell resolution of a map in the per-map pipeline (the
+// single-map default, nside / 32, at most kMaxSortNside).
 static int sort_nside(int nside)
 {
-    const int nc = nside >= 8 ? nside / 8 : 1;
+    const int nc = default_cell_nside(nside);
     return nc < kMaxSortNside ? nc : kMaxSortNside;
 }
 
@@ -148,7 +148,7 @@ static hipError_t ensure_pipeline(dmann_batch *b, int64_t n)
         b->recs_cap = cap;
     }
     if (b->acc != kAccF64 && !b->gc.hist) {
-        const int64_t nbins = 12LL * b->nside_c_max * b->nside_c_max * kSizeBuckets;
+        const int64_t nbins = 12LL * b->nside_c_max * b->nside_c_max * kSizeBuckets * 16;
         if ((e = hipMalloc(&b->gc.hist, nbins * sizeof(unsigned))) != hipSuccess) return e;
         if ((e = hipMalloc(&b->gc.sums, ((nbins + 1023) / 1024) * sizeof(unsigned))) != hipSuccess) return e;
         if ((e = hipMalloc(&b->gc.work, sizeof(unsigned))) != hipSuccess) return e;

@@ -731,27 +731,6 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
 
 // ---------------------------------------------------------------- sorting
 
-// Sort key of a medium-class record: coarse sky cell (RING pixel at
-// nside_c) times kSizeBuckets plus the footprint-size class.
-__device__ __forceinline__ unsigned sort_key(const DiscRec &r, int nside_c)
-{
-    const int lg = r.est > 0 ? 31 - __clz(r.est) : 0;
-    const int bucket = lg < 6 ? 0 : (lg - 6 > kSizeBuckets - 1 ? kSizeBuckets - 1 : lg - 6);
-    return (unsigned)ang2pix(nside_c, r.theta, r.phi) * (unsigned)kSizeBuckets + (unsigned)bucket;
-}
-
-// Sort key of every medium-class record and the keys' histogram.
-__global__ void k_sort_keys(const DiscRec *__restrict__ recs, const unsigned *__restrict__ counts, int nside_c,
-                            unsigned *__restrict__ keys, unsigned *__restrict__ hist)
-{
-    const unsigned n = counts[0];
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const unsigned key = sort_key(recs[i], nside_c);
-        keys[i] = key;
-        atomicAdd(&hist[key], 1u);
-    }
-}
-
 // Bits 0..11 of v spread to the even bit positions 0..22.
 __device__ __forceinline__ unsigned spread_bits12(unsigned v)
 {
@@ -814,6 +793,27 @@ __device__ __forceinline__ unsigned radix_key(const DiscRec &r, int nf, int sub)
     const unsigned cell = nest_cell(nf, r.theta, r.phi);
     const unsigned lo = cell & ((1u << (2 * sub)) - 1u);
     return ((((cell >> (2 * sub)) << 4) | bucket) << (2 * sub)) | lo;
+}
+
+// Key of the counting sort (the graph-captured batch pipeline, whose
+// histogram must stay small): the radix key's three parts with the fine
+// cell at 4 nc, i.e. the 4 x 4 NESTED sub-cells of a coarse cell at nc --
+// 12 nc^2 x 16 x 16 bins.
+__device__ __forceinline__ unsigned count_key(const DiscRec &r, int nc)
+{
+    return radix_key(r, 4 * nc, 2);
+}
+
+// Counting-sort key of every medium-class record and the keys' histogram.
+__global__ void k_sort_keys(const DiscRec *__restrict__ recs, const unsigned *__restrict__ counts, int nc,
+                            unsigned *__restrict__ keys, unsigned *__restrict__ hist)
+{
+    const unsigned n = counts[0];
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned key = count_key(recs[i], nc);
+        keys[i] = key;
+        atomicAdd(&hist[key], 1u);
+    }
 }
 
 // Keys of the stable radix sort: records below the medium count get their
@@ -959,10 +959,13 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
                                                        (size_t)nrec, 0, bits, st);
         if (e != hipSuccess) return e;
     } else {
-        const int nbins = 12 * gc.nside_c * gc.nside_c * kSizeBuckets;
+        // coarse cells at nc (a power of two, at most kMaxSortNside), fine
+        // cells at 4 nc
+        const int nc = nest_nside(gc.nside_c < kMaxSortNside ? gc.nside_c : kMaxSortNside);
+        const int nbins = 12 * nc * nc * kSizeBuckets * 16;
         const int nb = (nbins + 1023) / 1024;
         k_zero_u32<<<nb < 4096 ? nb : 4096, 256, 0, st>>>(gc.hist, nbins);
-        k_sort_keys<<<gc.sort_blocks, 256, 0, st>>>(recs, counts, gc.nside_c, gc.keys, gc.hist);
+        k_sort_keys<<<gc.sort_blocks, 256, 0, st>>>(recs, counts, nc, gc.keys, gc.hist);
         k_scan_blocks<<<nb, 256, 0, st>>>(gc.hist, nbins, gc.sums);
         k_scan_sums<<<1, 256, 0, st>>>(gc.sums, nb);
         k_scan_add<<<nb, 256, 0, st>>>(gc.hist, nbins, gc.sums);
@@ -970,10 +973,10 @@ hipError_t launch_group(int acc, const DiscRec *recs, unsigned *counts, int nsid
     }
     if (sorted) (void)hipEventRecord(sorted, st);
     // Groups in NESTED order are compact, and consecutive ones overlap:
-    // spread them over the waves (at 1.1e9 particles, nside 2048, K = 32:
-    // 4868 ms spread against 5183 ms in order; the RING order of the
-    // counting sort was 1.6 % slower spread).
-    const unsigned spread = gc.sort_tmp ? 2654435761u : 1u;
+    // spread them over the waves (at 1.1e9 particles, nside 2048, K = 32,
+    // class-major key: 4868 ms spread against 5183 ms in order; the batched
+    // 64-observer workload: 9305 ms spread against 9557 ms in order).
+    const unsigned spread = 2654435761u;
     if (acc == kAccComp)
         k_disc_group<kAccComp><<<gc.group_blocks, 64, 0, st>>>(recs, gc.idx, counts, nside, pad, gc.k, map, stats, gc.work,
                                                                gc.tab_f, gc.tab_i, spread);

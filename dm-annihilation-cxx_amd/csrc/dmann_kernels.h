@@ -130,7 +130,7 @@ struct LaunchCfg {
 // (floor(log2 est) - 6, clamped to 0..15), so that a group of neighbours
 // also holds footprints of similar size and its window stays tight.
 constexpr int kSizeBuckets = 16;
-constexpr int kMaxSortNside = 1024; // counting sort (histogram of 12 nside_c^2 x 16 bins)
+constexpr int kMaxSortNside = 256;  // counting sort (histogram of 12 nside_c^2 x 256 bins)
 constexpr int kMaxNestNside = 4096; // radix sort (class and NESTED cell in 32 key bits)
 // Default group size.  The window's FP32 partial sums hold up to K deposits
 // per pixel; members deposit in ascending flux order, so small deposits are
@@ -163,7 +163,7 @@ struct GroupCfg {
     unsigned *iota;
     void *sort_tmp;
     size_t sort_tmp_bytes;
-    unsigned *hist;   // counting sort: 12 nside_c^2 kSizeBuckets bins
+    unsigned *hist;   // counting sort: 12 nside_c^2 x kSizeBuckets x 16 bins
     unsigned *sums;   // ceil(bins / 1024)
     unsigned *work;   // group counter
     // per-wave cache of the members' ring terms (group_blocks x k x 64
