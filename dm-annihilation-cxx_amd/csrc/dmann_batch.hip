@@ -118,7 +118,7 @@ static void batch_free(dmann_batch *b)
 }
 
 // Coarse sort-cell resolution of a map in the per-map pipeline (the
-// single-map default, nside / 32, at most kMaxSortNside).
+// single-map default, at most kMaxSortNside).
 static int sort_nside(int nside)
 {
     const int nc = default_cell_nside(nside);
@@ -172,6 +172,8 @@ static hipError_t enqueue(dmann_batch *b, const Particle *parts, int64_t n)
         for (int m = 0; m < b->nmaps; m++) {
             GroupCfg gc = b->gc;
             gc.nside_c = sort_nside(b->Pmap[m].nside);
+            const int ka = auto_group_k(n, 12LL * b->Pmap[m].nside * b->Pmap[m].nside);
+            if (gc.k > ka) gc.k = ka;
             if ((e = launch_render(b->acc, parts, n, b->Pmap[m], b->recs, b->counts, b->maps[m],
                                    b->d_stats + (size_t)m * kNumStats, b->cfg, b->stream, nullptr,
                                    b->acc == kAccF64 ? nullptr : &gc)) != hipSuccess)

@@ -322,8 +322,13 @@ static int render_launch(dmann_ctx *c, const Particle *parts, int64_t n, hipEven
         const int r = fix_prepare(c, parts, n);
         if (r != DMANN_OK) return r;
     }
+    GroupCfg gc = c->gc;
+    if (gc.k_auto && gc.k > 0) {
+        const int ka = auto_group_k(n, c->npix);
+        if (ka < gc.k) gc.k = ka;
+    }
     HIPCHK(c, launch_render(c->acc, parts, n, c->P, c->d_recs, c->d_counts, c->map, c->d_stats, c->cfg, c->stream, ev,
-                            &c->gc, c->acc == kAccFix ? &c->det : nullptr));
+                            &gc, c->acc == kAccFix ? &c->det : nullptr));
     return DMANN_OK;
 }
 
@@ -434,6 +439,7 @@ int dmann_create(dmann_ctx **out, const dmann_params *p, int64_t max_parts, int 
     c->gc.sort_blocks = dev_cus * 16;
     c->gc.group_blocks = dev_cus * DMANN_GROUP_BPC;
     if ((e = alloc_group(c, kDefaultGroupK, default_cell_nside(p->nside))) != hipSuccess) return bail("hipMalloc sort cells", e);
+    c->gc.k_auto = true;
     if (c->acc == kAccFix && (e = alloc_det(c)) != hipSuccess) return bail("hipMalloc reproducible-form buffers", e);
     if ((e = launch_clear(c->acc, c->map, np, c->d_stats, c->stream)) != hipSuccess) return bail("clear", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear sync", e);
@@ -482,6 +488,7 @@ int dmann_set_grouping(dmann_ctx *c, int k, int nside_c)
     FLUSH(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, alloc_group(c, k, nside_c));
+    c->gc.k_auto = false;
     return DMANN_OK;
 }
 

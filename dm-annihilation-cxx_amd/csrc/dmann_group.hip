@@ -4,9 +4,10 @@
 // channel (~2.6e11/s chip-wide) even when an instruction's 64 lanes hit
 // consecutive dwords (fewer when they spread over more cache lines), and
 // LDS atomics are no faster; with ~1e12 footprint pixels per map, one atomic per pixel is
-// the bottleneck.  Here the medium-class particles are first sorted by a
-// coarse sky cell (counting sort), then one wavefront takes K consecutive
-// particles (a group of neighbours):
+// the bottleneck.  Here the medium-class particles are first sorted by sky
+// cell and footprint-size class (radix sort, or a counting sort in the
+// graph-captured batch), then one wavefront takes K consecutive particles
+// (a group of neighbours), in ascending flux order:
 //   1. the normalisation pass (FP32 weights, FP64 sums): every (particle,
 //      ring) row of the group on a lane of its own, 64 rows at a time;
 //   2. the group's window -- the rings irmin..irmax of all members and the
@@ -17,7 +18,8 @@
 //   3. the band is flushed with one global atomic (form) per non-zero pixel.
 // A group's pixels are written ~K times less often than with per-pixel
 // atomics.  The window holds FP32 partial sums of at most K deposits per
-// pixel (one per member; error <= (K-1) u of the partial sum); the
+// pixel (one per member, the small ones first; error <= (K-1) u of the
+// partial sum); the
 // compensated form flushes them with the two-float atomic pair (exact
 // TwoSum), so the sum over groups -- 1e4-1e5 terms per pixel -- stays
 // compensated.

@@ -141,15 +141,25 @@ constexpr int kMaxNestNside = 4096; // radix sort (class and NESTED cell in 32 k
 // K = 32 gave 1.8e-7 / 5.6e-9); the grouped kernel took 4743 ms at K = 64
 // against 4904 ms at K = 32 (1.1e9 particles).
 constexpr int kDefaultGroupK = 64;
-// Default coarse cell resolution of the radix sort: nside / 32 (cells of
-// 32 x 32 map pixels).  Grouped kernel, K = 64, coarse nside 2048/c at
-// 1.1e9 particles: c = 8 4925 ms, 16 4752, 32 4641, 64 4601, 128 4578,
-// 2048 (one cell per face) 4854; 1e7 particles at nside 1024: c = 8 233 ms,
-// 16 221, 32 210.
-inline int default_cell_nside(int nside) { return nside >= 32 ? nside / 32 : 1; }
+// Group size of a render of n particles into a map of npix pixels when the
+// caller has not chosen one: K = 64 where the particles are dense (at
+// least 8 per pixel), else 32.  On 1e7 particles (NFW + 100 subhalos) at
+// nside 1024, 0.8 per pixel, K = 32 took 197 ms in the grouped kernel
+// against 210 ms at K = 64 (nside 512: 58 against 59 ms); at 1.1e9
+// particles, nside 2048, 22 per pixel, K = 64 was 2.5 % faster.
+inline int auto_group_k(int64_t n, int64_t npix) { return n >= 8 * npix ? 64 : 32; }
+// Default coarse cell resolution of the sort: nside_c = 32 (or the map's
+// nside below 32), cells of 1.8 degrees whatever the map's nside.  Grouped
+// kernel at 1.1e9 particles, nside 2048, K = 64, coarse nside 256 / 128 /
+// 64 / 32 / 16 / 1: 4925 / 4752 / 4641 / 4601 / 4578 / 4854 ms; on 1e7
+// particles at nside 256 / 512 / 1024 a coarse nside of 32 was as fast as
+// or faster than nside / 32 (20.0 / 58.0 / 196 ms against 20.9 / 58.1 /
+// 197 ms).
+inline int default_cell_nside(int nside) { return nside < 32 ? nside : 32; }
 
 struct GroupCfg {
     int k;            // particles per group (0: per-particle kernel instead)
+    bool k_auto;      // each render takes min(k, auto_group_k(n, npix))
     int nside_c;      // sort cell resolution
     int sort_blocks;
     int group_blocks; // single-wave workgroups

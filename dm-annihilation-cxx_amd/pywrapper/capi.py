@@ -312,10 +312,10 @@ class Projector:
 
     def set_grouping(self, k: int = 64, nside_c: int | None = None):
         """K neighbours per wavefront (0: one particle per wavefront), sorted
-        by coarse NESTED cell at nside_c (default nside / 32), then size
+        by coarse NESTED cell at nside_c (default min(nside, 32)), then size
         class, then NESTED cell at the map's nside."""
         ns = self.params.nside
-        _check(self._L.dmann_set_grouping(self._h, int(k), int(nside_c or max(1, ns // 32))), self._h)
+        _check(self._L.dmann_set_grouping(self._h, int(k), int(nside_c or min(ns, 32))), self._h)
 
     def timing(self, enable: bool):
         """Returns ({'prepass','large','sort','medium'}: summed ms, calls)

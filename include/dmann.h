@@ -211,7 +211,9 @@ int dmann_set_sky_gather(dmann_ctx *ctx, int blocks_per_cu);
  * then by NESTED cell at the map's nside, and K (1..64) neighbours share
  * one wavefront and an LDS window, cutting global atomics ~K-fold; k = 0
  * renders one particle per wavefront with per-pixel atomics.  Defaults
- * K = 64, nside_c = nside / 32.  Ignored by the FP64 form. */
+ * K = 64 (32 when a render has fewer than 8 particles per pixel) and
+ * nside_c = min(nside, 32); a K set here holds for every render.  Ignored
+ * by the FP64 form. */
 int dmann_set_grouping(dmann_ctx *ctx, int k, int nside_c);
 
 /* Uses `stream` (a hipStream_t, NULL = the null stream) for later calls. */
