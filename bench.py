@@ -41,7 +41,7 @@ EVALS_PER_CONTRIB = 2
 # counters read for k_finalize, a streaming kernel of known size (fetch x2,
 # write x1): profiles/k_disc_group_nfw_sub2000_near_1p1e9_nside2048_traffic.md
 MEASURED_TRAFFIC = {
-    ("nfw_sub2000_near_1p1e9", 2048, "compensated"): (1.2369e9 * 2.0 + 2.9188e9 * 1.0) * 1024,
+    ("nfw_sub2000_near_1p1e9", 2048, "compensated"): (1.2344e9 * 2.0 + 2.7896e9 * 1.0) * 1024,
 }
 
 
