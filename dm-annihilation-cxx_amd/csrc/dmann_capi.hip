@@ -793,6 +793,15 @@ int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q)
     return mism;
 }
 
+int64_t dmann_host_nest_cells(int nside, const double *theta, const double *phi, int64_t n, uint32_t *cells)
+{
+    if (nside < 1 || nside > kMaxNestNside || (nside & (nside - 1)) != 0 || n < 0 ||
+        (n > 0 && (!theta || !phi || !cells)))
+        return -1;
+    for (int64_t i = 0; i < n; ++i) cells[i] = nest_cell(nside, theta[i], phi[i]);
+    return n;
+}
+
 int64_t dmann_host_sky_check(const dmann_params *p, const dmann_particle *q)
 {
     if (!p || !q || p->nside < 1 || p->nside > kMaxNside) return -1;

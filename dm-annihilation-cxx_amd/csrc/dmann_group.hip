@@ -733,54 +733,6 @@ __global__ __launch_bounds__(64) DMANN_GROUP_ATTR void k_disc_group(const DiscRe
 
 // ---------------------------------------------------------------- sorting
 
-// Bits 0..11 of v spread to the even bit positions 0..22.
-__device__ __forceinline__ unsigned spread_bits12(unsigned v)
-{
-    v &= 0xfffu;
-    v = (v | (v << 8)) & 0x00ff00ffu;
-    v = (v | (v << 4)) & 0x0f0f0f0fu;
-    v = (v | (v << 2)) & 0x33333333u;
-    v = (v | (v << 1)) & 0x55555555u;
-    return v;
-}
-
-// NESTED pixel index at nside nc (a power of two, at most kMaxNestNside)
-// of the direction (theta, phi): base face and the bit-interleaved
-// position in it, the HEALPix NESTED scheme (Gorski et al. 2005, ApJ 622,
-// 759).  Consecutive indices form compact 2-D blocks on the sky (a Z-order
-// curve in each base face).  Only the grouping's locality depends on it.
-__device__ __forceinline__ unsigned nest_cell(int nc, double theta, double phi)
-{
-    const double z = cos(theta), za = fabs(z);
-    const double tt = fmodulo(phi * (1.0 / kHalfPi), 4.0);
-    int face, ix, iy;
-    if (za <= kTwoThird) {
-        const double t1 = nc * (0.5 + tt), t2 = nc * z * 0.75;
-        const int jp = (int)(t1 - t2), jm = (int)(t1 + t2);
-        const int ifp = jp / nc, ifm = jm / nc;
-        face = ifp == ifm ? (ifp | 4) : (ifp < ifm ? ifp : ifm + 8);
-        ix = jm & (nc - 1);
-        iy = nc - (jp & (nc - 1)) - 1;
-    } else {
-        const int ntt = (int)tt < 3 ? (int)tt : 3;
-        const double tp = tt - ntt;
-        const double tmp = nc * sqrt(3.0 * (1.0 - za));
-        int jp = (int)(tp * tmp), jm = (int)((1.0 - tp) * tmp);
-        jp = jp < nc - 1 ? jp : nc - 1;
-        jm = jm < nc - 1 ? jm : nc - 1;
-        if (z >= 0) {
-            face = ntt;
-            ix = nc - jm - 1;
-            iy = nc - jp - 1;
-        } else {
-            face = ntt + 8;
-            ix = jp;
-            iy = jm;
-        }
-    }
-    return (unsigned)face * (unsigned)nc * (unsigned)nc + (spread_bits12((unsigned)ix) | (spread_bits12((unsigned)iy) << 1));
-}
-
 // Key of the radix sort: the NESTED cell at the coarse nside nh, then the
 // footprint-size class, then the NESTED sub-cell at the fine nside nf =
 // nh 2^sub (NESTED indices nest: a cell at nh is the fine index >> 2 sub).
@@ -811,8 +763,12 @@ __global__ void k_sort_keys(const DiscRec *__restrict__ recs, const unsigned *__
                             unsigned *__restrict__ keys, unsigned *__restrict__ hist)
 {
     const unsigned n = counts[0];
+    // (keys are below 12 nc^2 x 256 by construction; the clamp keeps the
+    // histogram update in bounds whatever the record holds)
+    const unsigned top = 3072u * (unsigned)nc * (unsigned)nc - 1u;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const unsigned key = count_key(recs[i], nc);
+        const unsigned k = count_key(recs[i], nc);
+        const unsigned key = k < top ? k : top;
         keys[i] = key;
         atomicAdd(&hist[key], 1u);
     }

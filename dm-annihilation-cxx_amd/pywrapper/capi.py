@@ -72,7 +72,7 @@ EXPORTS = ("dmann_params_from_ini", "dmann_create", "dmann_render", "dmann_rende
            "dmann_set_stream", "dmann_synchronize", "dmann_get_map", "dmann_get_map_device",
            "dmann_reset", "dmann_get_stats", "dmann_npix", "dmann_destroy",
            "dmann_write_healpix_fits", "dmann_read_healpix_fits", "dmann_host_footprint", "dmann_host_ring_check",
-           "dmann_host_sky_check",
+           "dmann_host_sky_check", "dmann_host_nest_cells",
            "dmann_timing", "dmann_adjoint_device", "dmann_last_error", "dmann_set_tuning", "dmann_set_grouping",
            "dmann_batch_create", "dmann_batch_set_tuning", "dmann_batch_render_device", "dmann_batch_set_stream",
            "dmann_batch_synchronize", "dmann_batch_npix", "dmann_batch_map_device", "dmann_batch_get_map",
@@ -128,6 +128,8 @@ def load_library():
     L.dmann_host_ring_check.restype = i64
     L.dmann_host_sky_check.argtypes = [C.POINTER(DmannParams), vp]
     L.dmann_host_sky_check.restype = i64
+    L.dmann_host_nest_cells.argtypes = [C.c_int, vp, vp, i64, vp]
+    L.dmann_host_nest_cells.restype = i64
     L.dmann_adjoint_device.argtypes = [vp, vp, i64, vp, vp]
     L.dmann_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.dmann_set_grouping.argtypes = [vp, C.c_int, C.c_int]
@@ -442,6 +444,20 @@ def host_sky_check(params: DmannParams, rec) -> int:
     if r.size != 12:
         r = np.ascontiguousarray(np.asarray(rec, dtype=np.float32).reshape(12))
     return int(L.dmann_host_sky_check(C.byref(params), C.c_void_p(r.ctypes.data)))
+
+
+def host_nest_cells(nside: int, theta, phi) -> np.ndarray:
+    """HEALPix NESTED index at nside (a power of two) of the directions
+    (theta, phi), as the grouping's sort computes it (host build)."""
+    L = load_library()
+    t = np.ascontiguousarray(theta, dtype=np.float64)
+    f = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.empty(t.size, dtype=np.uint32)
+    r = L.dmann_host_nest_cells(int(nside), C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), t.size,
+                                C.c_void_p(out.ctypes.data))
+    if r != t.size:
+        raise ValueError(f"dmann_host_nest_cells: bad arguments (nside {nside})")
+    return out
 
 
 # ------------------------------------------------------------------ batch

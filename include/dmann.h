@@ -361,6 +361,12 @@ int64_t dmann_host_ring_check(const dmann_params *p, const dmann_particle *q);
  * for bad arguments. */
 int64_t dmann_host_sky_check(const dmann_params *p, const dmann_particle *q);
 
+/* Test hook (no counterpart in the reference): the HEALPix NESTED index at
+ * nside (a power of two, 1..4096) of n directions (theta, phi), as the
+ * grouping's sort computes it on the device, into cells.  Returns n, or -1
+ * for bad arguments. */
+int64_t dmann_host_nest_cells(int nside, const double *theta, const double *phi, int64_t n, uint32_t *cells);
+
 /* Allocation-free render.  No counterpart in the reference: for callers
  * that cannot allocate device memory -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc.  Renders n particles

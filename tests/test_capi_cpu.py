@@ -153,6 +153,59 @@ def test_sky_gather_indexing(pw, tmp_path, nside):
     assert checked >= 15
 
 
+def _ring_centres(nside):
+    """Directions (theta, phi) of the centres of all RING pixels at nside
+    (the HEALPix RING layout: polar caps of 4 i pixels on ring i, the
+    equatorial belt of 4 nside pixels per ring, alternately shifted by half
+    a pixel): test inputs."""
+    n = nside
+    p = np.arange(12 * n * n, dtype=np.int64)
+    z = np.empty(p.size)
+    phi = np.empty(p.size)
+    ncap = 2 * n * (n - 1)
+    north = p < ncap
+    i = np.floor((1 + np.sqrt(1 + 2 * p[north])) / 2).astype(np.int64)
+    j = p[north] - 2 * i * (i - 1) + 1
+    z[north] = 1 - i * i / (3.0 * n * n)
+    phi[north] = (j - 0.5) * np.pi / (2 * i)
+    belt = (p >= ncap) & (p < 12 * n * n - ncap)
+    q = p[belt] - ncap
+    i = q // (4 * n) + n
+    j = q % (4 * n) + 1
+    shift = (i - n + 1) % 2
+    z[belt] = 4.0 / 3.0 - 2.0 * i / (3.0 * n)
+    phi[belt] = (j - 1 + shift / 2.0) * np.pi / (2 * n)
+    south = p >= 12 * n * n - ncap
+    q = 12 * n * n - p[south]
+    i = np.floor((1 + np.sqrt(2 * q - 1)) / 2).astype(np.int64)
+    j = 4 * i + 1 - (q - 2 * i * (i - 1))
+    z[south] = -1 + i * i / (3.0 * n * n)
+    phi[south] = (j - 0.5) * np.pi / (2 * i)
+    return np.arccos(z), phi
+
+
+def test_nest_cells(pw):
+    """The NESTED index the grouping's sort keys use (host build of the
+    device code): a permutation of 0 .. 12 nside^2 - 1 over the RING pixel
+    centres at nside 1..128; at nside 1 the base faces (RING and NESTED
+    agree); at nside 2 the first RING pixel is NESTED 3 (the north corner
+    of face 0); and each pixel's index at nside / 2 is its index >> 2 (the
+    parent cell), which makes consecutive keys compact 2-D patches."""
+    for nside in (1, 2, 4, 8, 64, 128):
+        th, ph = _ring_centres(nside)
+        cells = pw.capi.host_nest_cells(nside, th, ph)
+        assert np.array_equal(np.sort(cells), np.arange(12 * nside * nside, dtype=np.uint32)), nside
+        if nside == 1:
+            assert np.array_equal(cells, np.arange(12, dtype=np.uint32))
+        if nside == 2:
+            assert cells[0] == 3
+        if nside > 1:
+            parent = pw.capi.host_nest_cells(nside // 2, th, ph)
+            assert np.array_equal(cells >> 2, parent), nside
+    with pytest.raises(ValueError):
+        pw.capi.host_nest_cells(3, np.zeros(1), np.zeros(1))
+
+
 @pytest.mark.parametrize("nside", [256, 2048])
 def test_polar_series_form_matches_oracle(pw, orc, tmp_path, nside):
     """Footprints of radius <= 0.5 that are too close to a pole for the fast
