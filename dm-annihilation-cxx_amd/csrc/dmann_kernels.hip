@@ -594,11 +594,22 @@ __global__ __launch_bounds__(256) void k_adjoint(const Particle *__restrict__ pa
                 const int nwin = (g.nrings + 63) / 64;
                 for (int win = 0; win < nwin; ++win) {
                     const int Tw = setup_window<true>(g, P.nside, win, L, lane, 0);
+                    // the gradient map value of the lane's next pixel is
+                    // loaded while this one is computed (the loop is bound
+                    // by that load's latency, not by its FP64 work)
                     int j = 0;
+                    if (lane < Tw)
+                        while (L.pre_end[j] <= lane) ++j;
+                    double gp_next = lane < Tw ? gmap[pix_addr(L.a[j], lane)] : 0.0;
                     for (int k = lane; k < Tw; k += 64) {
-                        while (L.pre_end[j] <= k) ++j;
-                        const double4 dd = L.d[j];
-                        const int4 a = L.a[j];
+                        const double gp_k = gp_next;
+                        const int jk = j;
+                        if (k + 64 < Tw) {
+                            while (L.pre_end[j] <= k + 64) ++j;
+                            gp_next = gmap[pix_addr(L.a[j], k + 64)];
+                        }
+                        const double4 dd = L.d[jk];
+                        const int4 a = L.a[jk];
                         const double phic = ((double)(k - a.w) + dd.z) * dd.w;
                         double sp, cp;
                         sincos(phic, &sp, &cp);
@@ -610,7 +621,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const Particle *__restrict__ pa
                         const double sinp = sqrt(cx * cx + cy * cy + cz * cz);
                         const double psi = atan2(sinp, cosp);
                         const double w = exp(-c * psi * psi / (r * r));
-                        const double gp = gmap[pix_addr(a, k)] * inv_dO;
+                        const double gp = gp_k * inv_dO;
                         const double dwdr = w * 2.0 * c * psi * psi / (r * r * r);
                         const double ratio = sinp > 1e-300 ? psi / sinp : 1.0;
                         const double fv = w * 2.0 * c * ratio / (r * r);
