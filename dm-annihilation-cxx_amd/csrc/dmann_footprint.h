@@ -376,7 +376,7 @@ DMANN_HD int seg_slot(int c0, int t, int nr, int s0)
 // lies farther than a bound on the FP32 error (relative 1e-6 of dphi) from
 // an integer, otherwise dphi is recomputed in FP64 exactly as ring_setup
 // does.  Only the FP32 weight terms are filled (not z, s, shift); the
-// column step is an FP32 quotient and the offset of column kc is its
+// column step is 2 pi times an FP32 reciprocal and the offset of column kc is its
 // fraction of a column (|.| <= 1/2, from the same FP64 product as kc) times
 // the step, so no FP64 division is made.
 //
@@ -419,6 +419,16 @@ DMANN_HD float fast_sqrt(float x)
 #endif
 }
 
+// Reciprocal, the hardware instruction on the device (1 ulp).
+DMANN_HD float fast_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // asin(sqrt m) for m in [0, 1/2] (FP32): sqrt m times a degree-8
 // polynomial fitted to asin(sqrt m) / sqrt m on [0, 1/2] (Chebyshev least
 // squares, error 6e-9 relative; with the roundings about 2e-7).
@@ -458,7 +468,7 @@ DMANN_HD RingLite ring_lite(const PGeo &g, const RingGeo &G, int iz)
         // are checked against)
         const float xf = (float)x, yf = (float)ysq;
         const float R = fast_sqrt(fmaf(xf, xf, yf));
-        const float m = yf / (2.0f * R * (R + fabsf(xf)));
+        const float m = yf * fast_rcp(2.0f * R * (R + fabsf(xf)));
         const float a2 = 2.0f * asin_sqrt(m);
         const float df = xf >= 0.0f ? a2 : 3.14159274f - a2;
         double dphi = (double)df;
@@ -492,7 +502,7 @@ DMANN_HD RingLite ring_lite(const PGeo &g, const RingGeo &G, int iz)
         o.e1 = o.B;
         o.e2 = 0.0f;
     }
-    o.step = (float)kTwoPi / (float)o.nr;
+    o.step = (float)kTwoPi * fast_rcp((float)o.nr);
     o.offm = (float)(((double)kc + shift) - a) * o.step;
     o.kc_rel = kc - o.col0;
     return o;
