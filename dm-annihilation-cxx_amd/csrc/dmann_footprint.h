@@ -376,7 +376,7 @@ DMANN_HD int seg_slot(int c0, int t, int nr, int s0)
 // lies farther than a bound on the FP32 error (relative 1e-6 of dphi) from
 // an integer, otherwise dphi is recomputed in FP64 exactly as ring_setup
 // does.  Only the FP32 weight terms are filled (not z, s, shift); the
-// column step is 2 pi times an FP32 reciprocal and the offset of column kc is its
+// column step is an FP32 quotient and the offset of column kc is its
 // fraction of a column (|.| <= 1/2, from the same FP64 product as kc) times
 // the step, so no FP64 division is made.
 //
@@ -502,7 +502,7 @@ DMANN_HD RingLite ring_lite(const PGeo &g, const RingGeo &G, int iz)
         o.e1 = o.B;
         o.e2 = 0.0f;
     }
-    o.step = (float)kTwoPi * fast_rcp((float)o.nr);
+    o.step = (float)kTwoPi / (float)o.nr;
     o.offm = (float)(((double)kc + shift) - a) * o.step;
     o.kc_rel = kc - o.col0;
     return o;
